@@ -1,0 +1,84 @@
+// Device-side declarations shared by the HIP translation units and the
+// host runtime (which launches through these wrappers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "device_math.hpp"
+
+namespace gmd {
+
+// Grid geometry and constants passed by value to every kernel.
+struct Geo {
+    int nx, ny, nz;
+    long long sx, sxy, P;     // padded strides and point count
+    double dx, dy, dz, dx1, dy1, dz1;
+    double Lx, Ly, Lz, Ox, Oy, Oz;
+    int xPeri, yPeri, zPeri;
+    int epsilonProf, zUpdate, pressure;
+    double beta, maxError, dtMin;
+    DistParams dist;
+};
+
+__host__ __device__ inline long long gidx(const Geo &g, long long i, long long j, long long k)
+{
+    return i + g.sx * j + g.sxy * k;
+}
+
+// Device control block of a context: the step controller state lives in
+// device memory so steps can be enqueued without host round trips.
+struct Ctl {
+    double t, dt;           // time and dt of the next attempt
+    double dt_used, maxDelta;
+    double ratio, energy, jxb_b2, jmax;  // diagnostics of the last evaluated state
+    int cur;                // index of the current position buffer
+    int accepted, nan_seen, done, converged, stop_reason;
+    long long n_attempt, n_accept;
+    long long max_attempt, n_cycle;      // run limits (n_cycle counts accepted steps)
+    double stop_ratio;
+    int diag_valid;
+    int pad;
+};
+
+struct StepRecord {
+    double t, dt_used, maxDelta, ratio, energy, jxb_b2, jmax;
+    int accepted, nan_seen;
+};
+
+// ---- per-kernel path (FP64, reference call structure) -----------------
+void launch_b_jacb0(hipStream_t s, const Geo &g, const double *X, const double *B0, double *B, double *det);
+void launch_bbound(hipStream_t s, const Geo &g, const double *X, const double *B0, double *B, double *det);
+void launch_set_peri(hipStream_t s, const Geo &g, double *F);
+void launch_set_grid_peri(hipStream_t s, const Geo &g, double *X);
+void launch_jstokes(hipStream_t s, const Geo &g, const double *X, const double *B, double *J);
+void launch_gradp(hipStream_t s, const Geo &g, const double *X, const double *det, double *G);
+void launch_kk(hipStream_t s, const Geo &g, const Tableau *tab, int n, double dt, const double *X,
+               const double *B, const double *J, const double *G, double *K, double *Xtmp);
+// x_new and per-block error maxima; returns the number of partials written.
+int launch_xnewstar(hipStream_t s, const Geo &g, const Tableau *tab, const double *X, const double *K,
+                    double *Xnew, double *partials, int *nanflag);
+// Final NaN-blind max of n partials into out[0].
+void launch_max_final(hipStream_t s, const double *partials, int n, double *out);
+// Diagnostics over inner points from B, J (padded SoA) and det:
+// out[0] = B2 (incl. dx dy dz), out[1] = JMax, out[2] = JxB_B2Max,
+// out[3] = JxB_JBMax.  `scratch` holds 4*blocks doubles.
+template <class R>
+void launch_diag(hipStream_t s, const Geo &g, const R *B, const R *J, const double *det, double *scratch,
+                 int max_blocks, double *out);
+
+// ---- layout conversion between host AoS padded and device SoA --------
+// dst[c*P + a] = src[3a + c] (src in double) with an optional hi/lo split.
+void launch_aos_to_soa_f64(hipStream_t s, long long P, const double *src, double *dst);
+void launch_soa_to_aos_f64(hipStream_t s, long long P, const double *src, double *dst);
+void launch_aos_to_soa_f32k(hipStream_t s, long long P, const double *src, float *hi, float *lo);
+void launch_soa_to_aos_f32k(hipStream_t s, long long P, const float *hi, const float *lo, double *dst);
+void launch_soa_to_aos_f32(hipStream_t s, long long P, const float *src, double *dst);
+void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float *dst);
+// Inner AoS (3N) <- padded SoA (3P), as J/gradP/kk are laid out on the host.
+template <class R>
+void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, double *dst);
+void launch_fill_f64(hipStream_t s, double *p, long long n, double v);
+void launch_fill_f32(hipStream_t s, float *p, long long n, float v);
+
+}  // namespace gmd

@@ -1,0 +1,848 @@
+// Host runtime of libglemur: device state of one relaxation (a context),
+// the per-kernel entry points of the reference's call structure, and the
+// fused step driver with its device-side controller.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gm_device.hpp"
+#include "gm_fused.hpp"
+#include "gm_internal.hpp"
+
+using gm::fail;
+
+#define HIPCHK(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(GM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+struct gm_ctx {
+    gm_params p{};
+    int prec = GM_PREC_F64;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    gmd::Geo geo{};
+    long long P = 0, N = 0;
+    // F64
+    double *X64[3] = {nullptr, nullptr, nullptr};
+    double *B0_64 = nullptr, *K64 = nullptr, *B64 = nullptr, *J64 = nullptr, *G64 = nullptr, *Xtmp64 = nullptr;
+    // F32K
+    float *Xh[2] = {nullptr, nullptr}, *Xl[2] = {nullptr, nullptr};
+    float *B0_32 = nullptr, *K32 = nullptr, *B32 = nullptr, *J32 = nullptr, *G32 = nullptr;
+    double *det = nullptr;
+    double *epsx = nullptr, *epsy = nullptr, *epsz = nullptr;
+    gmd::Ctl *ctl = nullptr;      // device
+    gmd::Ctl *ctlh = nullptr;     // pinned host mirror
+    double *part_delta = nullptr, *part_diag = nullptr;
+    int part_cap = 0;
+    gmd::Tableau *tab_dev = nullptr;
+    gmd::TableauF tabf{};
+    gmd::StepRecord *log = nullptr;
+    int log_cap = 4096;
+    double *red_out = nullptr, *red_host = nullptr;   // 4 doubles each
+    double *diag_scratch = nullptr;
+    int diag_blocks = 2048;
+    int *nanflag = nullptr;
+    int cur = 0;                  // host copy of ctl->cur (valid after syncs)
+    // tiles
+    int TX = 0, TY = 0, CZ = 0, tiles_x = 0, tiles_y = 0, nchunks = 0, nblocks = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_kind;
+    int ev_used = 0;
+    double tms[3] = {0, 0, 0};
+    long long tcount[3] = {0, 0, 0};
+    // graph of one batch
+    hipGraphExec_t gexec = nullptr;
+    int gbatch = 0;
+};
+
+namespace {
+
+template <class T>
+int dalloc(T **p, size_t n)
+{
+    if (n == 0) { *p = nullptr; return GM_OK; }
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(GM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return GM_OK;
+}
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+gmd::Geo make_geo(const gm_params &p)
+{
+    gmd::Geo g{};
+    g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
+    g.sx = p.nx + 2;
+    g.sxy = (long long)(p.nx + 2) * (p.ny + 2);
+    g.P = g.sxy * (p.nz + 2);
+    g.dx = p.dx; g.dy = p.dy; g.dz = p.dz;
+    g.dx1 = p.dx1; g.dy1 = p.dy1; g.dz1 = p.dz1;
+    g.Lx = p.Lx; g.Ly = p.Ly; g.Lz = p.Lz;
+    g.Ox = p.Ox; g.Oy = p.Oy; g.Oz = p.Oz;
+    g.xPeri = p.xPeri; g.yPeri = p.yPeri; g.zPeri = p.zPeri;
+    g.epsilonProf = p.epsilonProf; g.zUpdate = p.zUpdate; g.pressure = p.pressure;
+    g.beta = p.beta; g.maxError = p.maxError; g.dtMin = p.dtMin;
+    g.dist.code = p.initDistCode;
+    g.dist.shearA = p.initShearA; g.dist.shearB = p.initShearB; g.dist.shearK = p.initShearK;
+    g.dist.dx = p.dx; g.dist.dy = p.dy; g.dist.Ox = p.Ox; g.dist.Oy = p.Oy;
+    g.dist.Lx = p.Lx; g.dist.Ly = p.Ly;
+    return g;
+}
+
+// Cash-Karp tableau exactly as src/rungeKutta.cu:10-25 assigns it.
+void cash_karp(double b[42], double c[6], double cs[6])
+{
+    const double B[7][6] = {
+        {0, 0, 0, 0, 0, 0},
+        {0.2, 0, 0, 0, 0, 0},
+        {3 / 40., 9 / 40., 0, 0, 0, 0},
+        {0.3, -0.9, 1.2, 0, 0, 0},
+        {-11 / 54., 2.5, -70 / 27., 35 / 27., 0, 0},
+        {1631 / 55296., 175 / 512., 575 / 13824., 44275 / 110592., 253 / 4096., 0},
+        {0, 0, 0, 0, 0, 0}};
+    const double C[6] = {37 / 378., 0, 250 / 621., 125 / 594., 0, 512 / 1771.};
+    const double CS[6] = {2825 / 27648., 0, 18575 / 48384., 13525 / 55296., 277 / 14336., 0.25};
+    memcpy(b, B, sizeof B);
+    memcpy(c, C, sizeof C);
+    memcpy(cs, CS, sizeof CS);
+}
+
+bool any_peri(const gm_params &p) { return p.xPeri || p.yPeri || p.zPeri; }
+
+// Tile choice: the (TX+4)(TY+4) rect must fit the compiled LDS capacity;
+// among those, take the smallest loaded points per output point, then
+// split z so that the launch has about `target` workgroups.
+void auto_tiles(gm_ctx *c, int target)
+{
+    const int cap = gmd::fused_max_points();
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    double best = 1e30;
+    int bx = 0, by = 0;
+    for (int ty = 2; ty <= 32; ++ty)
+        for (int txm = 4; (txm + 4) * (ty + 4) <= cap; ++txm) {
+            int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
+            int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
+            if ((tx + 4) * (tyy + 4) > cap) continue;
+            double loaded = (double)tlx * tly * (tx + 4) * (tyy + 4);
+            double cost = loaded / ((double)nx * ny);
+            if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
+        }
+    c->TX = bx;
+    c->TY = by;
+    c->tiles_x = (nx + bx - 1) / bx;
+    c->tiles_y = (ny + by - 1) / by;
+    int cols = c->tiles_x * c->tiles_y;
+    int nch = std::max(1, std::min(nz / 8 > 0 ? nz / 8 : 1, (target + cols - 1) / cols));
+    c->CZ = (nz + nch - 1) / nch;
+    c->nchunks = (nz + c->CZ - 1) / c->CZ;
+    c->nblocks = cols * c->nchunks;
+}
+
+int ensure_partials(gm_ctx *c)
+{
+    if (c->nblocks <= c->part_cap) return GM_OK;
+    dfree(c->part_delta);
+    dfree(c->part_diag);
+    int r;
+    if ((r = dalloc(&c->part_delta, (size_t)c->nblocks)) || (r = dalloc(&c->part_diag, (size_t)4 * c->nblocks))) return r;
+    c->part_cap = c->nblocks;
+    return GM_OK;
+}
+
+gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_done)
+{
+    gmd::FusedArgs A{};
+    A.g = c->geo;
+    A.tab = c->tabf;
+    A.stage = stage;
+    A.TX = c->TX; A.TY = c->TY; A.CZ = c->CZ;
+    A.tiles_x = c->tiles_x; A.tiles_y = c->tiles_y; A.nchunks = c->nchunks; A.nblocks = c->nblocks;
+    A.emit = emit; A.diag = diag; A.check_done = check_done;
+    for (int i = 0; i < 3; ++i) A.X64[i] = c->X64[i];
+    A.B0_64 = c->B0_64; A.K64 = c->K64;
+    A.eB64 = c->B64; A.eJ64 = c->J64; A.eG64 = c->G64;
+    for (int i = 0; i < 2; ++i) { A.Xh[i] = c->Xh[i]; A.Xl[i] = c->Xl[i]; }
+    A.B0_32 = c->B0_32; A.K32 = c->K32;
+    A.eB32 = c->B32; A.eJ32 = c->J32; A.eG32 = c->G32;
+    A.edet = c->det;
+    A.epsx = c->epsx; A.epsy = c->epsy; A.epsz = c->epsz;
+    A.part_delta = c->part_delta; A.part_diag = c->part_diag;
+    A.ctl = c->ctl;
+    return A;
+}
+
+void ev_begin(gm_ctx *c, int kind)
+{
+    if (!c->timing) return;
+    if (c->ev_used + 2 > (int)c->ev.size()) {
+        for (int i = 0; i < 64; ++i) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            c->ev.push_back(e);
+            c->ev_kind.push_back(0);
+        }
+    }
+    c->ev_kind[c->ev_used] = kind;
+    (void)hipEventRecord(c->ev[c->ev_used], c->stream);
+    c->ev_used++;
+}
+void ev_end(gm_ctx *c)
+{
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev[c->ev_used], c->stream);
+    c->ev_used++;
+}
+// after a stream sync: fold recorded pairs into the totals
+void ev_collect(gm_ctx *c)
+{
+    for (int i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) {
+            c->tms[c->ev_kind[i]] += ms;
+            c->tcount[c->ev_kind[i]] += 1;
+        }
+    }
+    c->ev_used = 0;
+}
+
+// Enqueue one attempted step: stages 0..5 and the controller.
+void enqueue_step(gm_ctx *c, int diag, int check_done)
+{
+    int f32k = c->prec == GM_PREC_F32K;
+    for (int s = 0; s < 6; ++s) {
+        gmd::FusedArgs A = fused_args(c, s, 0, s == 0 ? diag : 0, check_done);
+        ev_begin(c, s < 5 ? 0 : 1);
+        gmd::launch_fused(c->stream, A, f32k);
+        ev_end(c);
+    }
+    gmd::CtlArgs C{};
+    C.ctl = c->ctl;
+    C.part_delta = c->part_delta;
+    C.part_diag = c->part_diag;
+    C.nblocks = c->nblocks;
+    C.diag = diag;
+    C.nbuf = f32k ? 2 : 3;
+    C.maxError = c->p.maxError;
+    C.dtMin = c->p.dtMin;
+    C.cell = c->p.dx * c->p.dy * c->p.dz;
+    C.log = c->log;
+    C.log_cap = c->log_cap;
+    ev_begin(c, 2);
+    gmd::launch_controller(c->stream, C);
+    ev_end(c);
+}
+
+int sync_ctl(gm_ctx *c)
+{
+    HIPCHK(hipMemcpyAsync(c->ctlh, c->ctl, sizeof(gmd::Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    c->cur = c->ctlh->cur;
+    return GM_OK;
+}
+int push_ctl(gm_ctx *c)
+{
+    HIPCHK(hipMemcpyAsync(c->ctl, c->ctlh, sizeof(gmd::Ctl), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GM_OK;
+}
+
+int need_f64(gm_ctx *c, const char *what)
+{
+    if (c->prec != GM_PREC_F64)
+        return fail(GM_ERR_UNSUPPORTED, std::string(what) + ": the per-kernel path runs in FP64 contexts only");
+    return GM_OK;
+}
+
+}  // namespace
+
+// ===================================================================== API
+extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
+{
+    if (!pp || !out) return fail(GM_ERR_ARG, "gm_create: null pointer");
+    if (precision != GM_PREC_F64 && precision != GM_PREC_F32K) return fail(GM_ERR_ARG, "unknown precision");
+    if (pp->nx < 2 || pp->ny < 2 || pp->nz < 2) return fail(GM_ERR_ARG, "grid needs at least 2 inner points per axis");
+    if (pp->inertia) return fail(GM_ERR_UNSUPPORTED, "inertia = t is not supported (magnetofrictional equation only)");
+    if (pp->jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver = t is not supported");
+    if (!gm::starts(pp->jMethod, "Stokes "))
+        return fail(GM_ERR_UNSUPPORTED, std::string("jMethod '") + std::string(pp->jMethod, 29) + "': only Stokes is built");
+    if (pp->pressure && !gm::starts(pp->pMethod, "Classic "))
+        return fail(GM_ERR_UNSUPPORTED, "pMethod: only Classic is built");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(GM_ERR_HIP, "no HIP device");
+    gm_ctx *c = new gm_ctx();
+    c->p = *pp;
+    c->prec = precision;
+    (void)hipGetDevice(&c->device);
+    c->geo = make_geo(c->p);
+    c->P = c->geo.P;
+    c->N = (long long)pp->nx * pp->ny * pp->nz;
+    const size_t P3 = 3 * (size_t)c->P;
+    int r = GM_OK;
+    auto bail = [&](int code) {
+        gm_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(GM_ERR_HIP, "stream"));
+    if (precision == GM_PREC_F64) {
+        for (int i = 0; i < 3; ++i)
+            if ((r = dalloc(&c->X64[i], P3))) return bail(r);
+        if ((r = dalloc(&c->B0_64, P3)) || (r = dalloc(&c->K64, 6 * P3)) || (r = dalloc(&c->B64, P3)) ||
+            (r = dalloc(&c->J64, P3)) || (r = dalloc(&c->G64, P3)))
+            return bail(r);
+        (void)hipMemset(c->K64, 0, 6 * P3 * sizeof(double));
+        (void)hipMemset(c->J64, 0, P3 * sizeof(double));
+        (void)hipMemset(c->G64, 0, P3 * sizeof(double));
+    } else {
+        for (int i = 0; i < 2; ++i)
+            if ((r = dalloc(&c->Xh[i], P3)) || (r = dalloc(&c->Xl[i], P3))) return bail(r);
+        if ((r = dalloc(&c->B0_32, P3)) || (r = dalloc(&c->K32, 5 * P3)) || (r = dalloc(&c->B32, P3)) ||
+            (r = dalloc(&c->J32, P3)) || (r = dalloc(&c->G32, P3)))
+            return bail(r);
+        (void)hipMemset(c->K32, 0, 5 * P3 * sizeof(float));
+        (void)hipMemset(c->J32, 0, P3 * sizeof(float));
+        (void)hipMemset(c->G32, 0, P3 * sizeof(float));
+    }
+    if ((r = dalloc(&c->det, (size_t)c->P))) return bail(r);
+    (void)hipMemset(c->det, 0, c->P * sizeof(double));
+    // friction profile factors (src/rungeKutta.cu:180-183), one table per axis
+    std::vector<double> ex(pp->nx), ey(pp->ny), ez(pp->nz);
+    for (int i = 0; i < pp->nx; ++i) ex[i] = gmd::eps_axis(i, pp->nx);
+    for (int i = 0; i < pp->ny; ++i) ey[i] = gmd::eps_axis(i, pp->ny);
+    for (int i = 0; i < pp->nz; ++i) ez[i] = gmd::eps_axis(i, pp->nz);
+    if ((r = dalloc(&c->epsx, ex.size())) || (r = dalloc(&c->epsy, ey.size())) || (r = dalloc(&c->epsz, ez.size())))
+        return bail(r);
+    (void)hipMemcpy(c->epsx, ex.data(), ex.size() * 8, hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->epsy, ey.data(), ey.size() * 8, hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->epsz, ez.data(), ez.size() * 8, hipMemcpyHostToDevice);
+    // tableaux
+    gmd::Tableau tab{};
+    cash_karp(tab.b, tab.c, tab.cs);
+    memcpy(c->tabf.b, tab.b, sizeof tab.b);
+    memcpy(c->tabf.c, tab.c, sizeof tab.c);
+    memcpy(c->tabf.cs, tab.cs, sizeof tab.cs);
+    for (int s = 0; s < 6; ++s)
+        for (int m = 0; m < 6; ++m) c->tabf.db[s * 6 + m] = tab.b[m + (s + 1) * 6] - tab.b[m + s * 6];
+    for (int m = 0; m < 6; ++m) c->tabf.ce[m] = tab.c[m] - tab.cs[m];
+    if ((r = dalloc(&c->tab_dev, 1))) return bail(r);
+    (void)hipMemcpy(c->tab_dev, &tab, sizeof tab, hipMemcpyHostToDevice);
+    // control block, log, reductions
+    if ((r = dalloc(&c->ctl, 1))) return bail(r);
+    if (hipHostMalloc((void **)&c->ctlh, sizeof(gmd::Ctl), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(GM_ERR_HIP, "hipHostMalloc"));
+    memset(c->ctlh, 0, sizeof(gmd::Ctl));
+    c->ctlh->dt = pp->dt0;
+    c->ctlh->n_cycle = 1LL << 62;
+    c->ctlh->max_attempt = 1LL << 62;
+    if ((r = dalloc(&c->log, (size_t)c->log_cap))) return bail(r);
+    if ((r = dalloc(&c->red_out, 8)) || (r = dalloc(&c->diag_scratch, (size_t)4 * c->diag_blocks)) ||
+        (r = dalloc(&c->nanflag, 1)))
+        return bail(r);
+    if (hipHostMalloc((void **)&c->red_host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(GM_ERR_HIP, "hipHostMalloc"));
+    auto_tiles(c, 512);
+    if ((r = ensure_partials(c))) return bail(r);
+    if ((r = push_ctl(c))) return bail(r);
+    *out = c;
+    return GM_OK;
+}
+
+extern "C" int gm_destroy(gm_ctx *c)
+{
+    if (!c) return GM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    for (int i = 0; i < 3; ++i) dfree(c->X64[i]);
+    dfree(c->B0_64); dfree(c->K64); dfree(c->B64); dfree(c->J64); dfree(c->G64); dfree(c->Xtmp64);
+    for (int i = 0; i < 2; ++i) { dfree(c->Xh[i]); dfree(c->Xl[i]); }
+    dfree(c->B0_32); dfree(c->K32); dfree(c->B32); dfree(c->J32); dfree(c->G32);
+    dfree(c->det); dfree(c->epsx); dfree(c->epsy); dfree(c->epsz);
+    dfree(c->ctl); dfree(c->part_delta); dfree(c->part_diag); dfree(c->tab_dev); dfree(c->log);
+    dfree(c->red_out); dfree(c->diag_scratch); dfree(c->nanflag);
+    if (c->ctlh) (void)hipHostFree(c->ctlh);
+    if (c->red_host) (void)hipHostFree(c->red_host);
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GM_OK;
+}
+
+extern "C" int gm_upload(gm_ctx *c, const double *B0, const double *xb, double t, double dt)
+{
+    if (!c || !B0 || !xb) return fail(GM_ERR_ARG, "gm_upload: null pointer");
+    const size_t P3 = 3 * (size_t)c->P;
+    double *stage = nullptr;
+    int r;
+    if ((r = dalloc(&stage, P3))) return r;
+    auto s = c->stream;
+    HIPCHK(hipMemcpyAsync(stage, xb, P3 * 8, hipMemcpyHostToDevice, s));
+    if (c->prec == GM_PREC_F64) {
+        for (int i = 0; i < 3; ++i) gmd::launch_aos_to_soa_f64(s, c->P, stage, c->X64[i]);
+    } else {
+        for (int i = 0; i < 2; ++i) gmd::launch_aos_to_soa_f32k(s, c->P, stage, c->Xh[i], c->Xl[i]);
+    }
+    HIPCHK(hipMemcpyAsync(stage, B0, P3 * 8, hipMemcpyHostToDevice, s));
+    if (c->prec == GM_PREC_F64) {
+        gmd::launch_aos_to_soa_f64(s, c->P, stage, c->B0_64);
+        // B starts as a copy of B0 (src/main.cu:195); ghost edges keep it
+        HIPCHK(hipMemcpyAsync(c->B64, c->B0_64, P3 * 8, hipMemcpyDeviceToDevice, s));
+    } else {
+        gmd::launch_aos_to_soa_f32(s, c->P, stage, c->B0_32);
+        HIPCHK(hipMemcpyAsync(c->B32, c->B0_32, P3 * 4, hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    dfree(stage);
+    gmd::Ctl &h = *c->ctlh;
+    memset(&h, 0, sizeof h);
+    h.t = t;
+    h.dt = dt;
+    h.cur = 0;
+    h.n_cycle = 1LL << 62;
+    h.max_attempt = 1LL << 62;
+    c->cur = 0;
+    return push_ctl(c);
+}
+
+extern "C" int gm_get_time(gm_ctx *c, double *t, double *dt)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    int r = sync_ctl(c);
+    if (r) return r;
+    if (t) *t = c->ctlh->t;
+    if (dt) *dt = c->ctlh->dt;
+    return GM_OK;
+}
+
+extern "C" int gm_set_time(gm_ctx *c, double t, double dt)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    int r = sync_ctl(c);
+    if (r) return r;
+    c->ctlh->t = t;
+    c->ctlh->dt = dt;
+    return push_ctl(c);
+}
+
+extern "C" int gm_synchronize(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GM_OK;
+}
+
+extern "C" int gm_download(gm_ctx *c, int field, double *host)
+{
+    if (!c || !host) return fail(GM_ERR_ARG, "gm_download: null pointer");
+    int r = sync_ctl(c);
+    if (r) return r;
+    const size_t P3 = 3 * (size_t)c->P, N3 = 3 * (size_t)c->N;
+    auto s = c->stream;
+    bool f64 = c->prec == GM_PREC_F64;
+    double *tmp = nullptr;
+    size_t n = 0;
+    switch (field) {
+    case GM_FIELD_XB:
+    case GM_FIELD_XB_NEW:
+    case GM_FIELD_B0:
+    case GM_FIELD_B:
+    case GM_FIELD_XB_LO: {
+        n = P3;
+        if ((r = dalloc(&tmp, n))) return r;
+        int idx = c->cur;
+        if (field == GM_FIELD_XB_NEW) idx = f64 ? (c->cur + 1) % 3 : c->cur ^ 1;
+        if (field == GM_FIELD_XB || field == GM_FIELD_XB_NEW) {
+            if (f64) gmd::launch_soa_to_aos_f64(s, c->P, c->X64[idx], tmp);
+            else gmd::launch_soa_to_aos_f32k(s, c->P, c->Xh[idx], c->Xl[idx], tmp);
+        } else if (field == GM_FIELD_XB_LO) {
+            if (f64) (void)hipMemsetAsync(tmp, 0, n * 8, s);
+            else gmd::launch_soa_to_aos_f32(s, c->P, c->Xl[idx], tmp);
+        } else if (field == GM_FIELD_B0) {
+            if (f64) gmd::launch_soa_to_aos_f64(s, c->P, c->B0_64, tmp);
+            else gmd::launch_soa_to_aos_f32(s, c->P, c->B0_32, tmp);
+        } else {
+            if (f64) gmd::launch_soa_to_aos_f64(s, c->P, c->B64, tmp);
+            else gmd::launch_soa_to_aos_f32(s, c->P, c->B32, tmp);
+        }
+        break;
+    }
+    case GM_FIELD_DETJAC:
+        HIPCHK(hipMemcpyAsync(host, c->det, c->P * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return GM_OK;
+    case GM_FIELD_J:
+    case GM_FIELD_GRADP: {
+        n = N3;
+        if ((r = dalloc(&tmp, n))) return r;
+        if (f64) gmd::launch_padded_to_inner_aos<double>(s, c->geo, field == GM_FIELD_J ? c->J64 : c->G64, tmp);
+        else gmd::launch_padded_to_inner_aos<float>(s, c->geo, field == GM_FIELD_J ? c->J32 : c->G32, tmp);
+        break;
+    }
+    case GM_FIELD_KK: {
+        n = 6 * N3;
+        if ((r = dalloc(&tmp, n))) return r;
+        (void)hipMemsetAsync(tmp, 0, n * 8, s);
+        for (int m = 0; m < (f64 ? 6 : 5); ++m) {
+            if (f64) gmd::launch_padded_to_inner_aos<double>(s, c->geo, c->K64 + m * P3, tmp + m * N3);
+            else gmd::launch_padded_to_inner_aos<float>(s, c->geo, c->K32 + m * P3, tmp + m * N3);
+        }
+        break;
+    }
+    default:
+        return fail(GM_ERR_ARG, "unknown field");
+    }
+    hipError_t e = hipMemcpyAsync(host, tmp, n * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(tmp);
+    if (e != hipSuccess) return fail(GM_ERR_HIP, std::string("download: ") + hipGetErrorString(e));
+    return GM_OK;
+}
+
+// ------------------------------------------------------- per-kernel path
+extern "C" int gm_b_jacb0(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_b_jacb0")) return r;
+    gmd::launch_b_jacb0(c->stream, c->geo, c->X64[c->cur], c->B0_64, c->B64, c->det);
+    return GM_OK;
+}
+extern "C" int gm_set_bbound(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_set_bbound")) return r;
+    gmd::launch_bbound(c->stream, c->geo, c->X64[c->cur], c->B0_64, c->B64, c->det);
+    return GM_OK;
+}
+extern "C" int gm_set_peri(gm_ctx *c, int field)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_set_peri")) return r;
+    if (field != GM_FIELD_B) return fail(GM_ERR_ARG, "gm_set_peri: field must be GM_FIELD_B");
+    gmd::launch_set_peri(c->stream, c->geo, c->B64);
+    return GM_OK;
+}
+extern "C" int gm_set_grid_peri(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_set_grid_peri")) return r;
+    gmd::launch_set_grid_peri(c->stream, c->geo, c->X64[c->cur]);
+    return GM_OK;
+}
+extern "C" int gm_current(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_current")) return r;
+    gmd::launch_jstokes(c->stream, c->geo, c->X64[c->cur], c->B64, c->J64);
+    return GM_OK;
+}
+extern "C" int gm_gradp(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_gradp")) return r;
+    gmd::launch_gradp(c->stream, c->geo, c->X64[c->cur], c->det, c->G64);
+    return GM_OK;
+}
+extern "C" int gm_rk_kk(gm_ctx *c, int n, double dt)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_rk_kk")) return r;
+    if (n < 0 || n > 5) return fail(GM_ERR_ARG, "stage index must be 0..5");
+    const size_t P3 = 3 * (size_t)c->P;
+    if (!c->Xtmp64) {
+        int r = dalloc(&c->Xtmp64, P3);
+        if (r) return r;
+    }
+    // xb_tmp starts as xb so its ghost layers match (src/main.cu:202)
+    HIPCHK(hipMemcpyAsync(c->Xtmp64, c->X64[c->cur], P3 * 8, hipMemcpyDeviceToDevice, c->stream));
+    gmd::launch_kk(c->stream, c->geo, c->tab_dev, n, dt, c->X64[c->cur], c->B64, c->J64, c->G64, c->K64, c->Xtmp64);
+    if (any_peri(c->p)) gmd::launch_set_grid_peri(c->stream, c->geo, c->Xtmp64);
+    HIPCHK(hipMemcpyAsync(c->X64[c->cur], c->Xtmp64, P3 * 8, hipMemcpyDeviceToDevice, c->stream));
+    return GM_OK;
+}
+extern "C" int gm_rk_xnewstar(gm_ctx *c, double *maxDelta)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_rk_xnewstar")) return r;
+    long long nb = (c->N + 255) / 256;
+    double *parts = nullptr;
+    int r = dalloc(&parts, (size_t)nb);
+    if (r) return r;
+    (void)hipMemsetAsync(c->nanflag, 0, sizeof(int), c->stream);
+    double *xnew = c->X64[(c->cur + 1) % 3];
+    // xb_new starts as xb (src/main.cu:209) so its ghost layers match
+    HIPCHK(hipMemcpyAsync(xnew, c->X64[c->cur], 3 * (size_t)c->P * 8, hipMemcpyDeviceToDevice, c->stream));
+    int n = gmd::launch_xnewstar(c->stream, c->geo, c->tab_dev, c->X64[c->cur], c->K64, xnew, parts, c->nanflag);
+    if (any_peri(c->p)) gmd::launch_set_grid_peri(c->stream, c->geo, xnew);
+    gmd::launch_max_final(c->stream, parts, n, c->red_out);
+    HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(parts);
+    if (maxDelta) *maxDelta = c->red_host[0];
+    return GM_OK;
+}
+extern "C" int gm_accept(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    int r = sync_ctl(c);
+    if (r) return r;
+    c->ctlh->cur = c->prec == GM_PREC_F64 ? (c->ctlh->cur + 1) % 3 : c->ctlh->cur ^ 1;
+    c->cur = c->ctlh->cur;
+    return push_ctl(c);
+}
+
+extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
+{
+    if (!c || !red) return fail(GM_ERR_ARG, "gm_reductions: null pointer");
+    if (c->prec == GM_PREC_F64)
+        gmd::launch_diag<double>(c->stream, c->geo, c->B64, c->J64, c->det, c->diag_scratch, c->diag_blocks, c->red_out);
+    else
+        gmd::launch_diag<float>(c->stream, c->geo, c->B32, c->J32, c->det, c->diag_scratch, c->diag_blocks, c->red_out);
+    HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 4 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    memset(red, 0, sizeof *red);
+    red->B2 = c->red_host[0];
+    red->JMax = c->red_host[1];
+    red->JxB_B2Max = c->red_host[2];
+    red->JxB_JBMax = c->red_host[3];
+    red->B2res = gm_init_residual(&c->p);
+    return GM_OK;
+}
+
+// ------------------------------------------------------------ fused path
+static int check_fused(gm_ctx *c)
+{
+    if (any_peri(c->p))
+        return fail(GM_ERR_UNSUPPORTED, "periodic boundaries run through the per-kernel path (gm_b_jacb0 ...)");
+    return GM_OK;
+}
+
+extern "C" int gm_fields(gm_ctx *c)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = check_fused(c)) return r;
+    gmd::FusedArgs A = fused_args(c, -1, 1, 0, 0);
+    gmd::launch_fused(c->stream, A, c->prec == GM_PREC_F32K);
+    HIPCHK(hipGetLastError());
+    return GM_OK;
+}
+
+static void fill_info(gm_ctx *c, gm_step_info *info)
+{
+    const gmd::Ctl &h = *c->ctlh;
+    info->accepted = h.accepted;
+    info->nan_seen = h.nan_seen;
+    info->stopped = h.done && !h.converged;
+    info->converged = h.converged;
+    info->dt_used = h.dt_used;
+    info->maxDelta = h.maxDelta;
+    info->t = h.t;
+    info->dt = h.dt;
+    info->n_attempt = h.n_attempt;
+    info->n_accept = h.n_accept;
+}
+
+extern "C" int gm_step(gm_ctx *c, gm_step_info *info)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = check_fused(c)) return r;
+    int r = sync_ctl(c);
+    if (r) return r;
+    gmd::Ctl &h = *c->ctlh;
+    h.done = 0;
+    h.converged = 0;
+    h.stop_reason = 0;
+    h.nan_seen = 0;
+    h.stop_ratio = 0;
+    h.n_cycle = 1LL << 62;
+    h.max_attempt = 1LL << 62;
+    if ((r = push_ctl(c))) return r;
+    enqueue_step(c, 1, 1);
+    HIPCHK(hipGetLastError());
+    if ((r = sync_ctl(c))) return r;
+    if (info) fill_info(c, info);
+    if (h.nan_seen) return fail(GM_ERR_NAN, "NaN in the error estimate (the reference's max would skip it)");
+    return GM_OK;
+}
+
+extern "C" int gm_run(gm_ctx *c, const gm_run_opts *o, gm_run_stats *st)
+{
+    if (!c || !o) return fail(GM_ERR_ARG, "gm_run: null pointer");
+    if (int r = check_fused(c)) return r;
+    int r = sync_ctl(c);
+    if (r) return r;
+    gmd::Ctl &h = *c->ctlh;
+    h.done = 0;
+    h.converged = 0;
+    h.stop_reason = 0;
+    h.nan_seen = 0;
+    h.diag_valid = 0;
+    h.stop_ratio = o->stop_ratio;
+    long long ncyc = o->n_cycle > 0 ? o->n_cycle : c->p.nCycle;
+    h.n_cycle = h.n_accept + ncyc;
+    h.max_attempt = h.n_attempt + (o->max_attempts > 0 ? o->max_attempts : (1LL << 40));
+    if ((r = push_ctl(c))) return r;
+    int batch = std::max(1, o->batch);
+    bool graph = o->use_graph && !c->timing;
+    auto t0 = std::chrono::steady_clock::now();
+    if (graph && (!c->gexec || c->gbatch != batch)) {
+        if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < batch; ++i) enqueue_step(c, 1, 1);
+        HIPCHK(hipStreamEndCapture(c->stream, &gr));
+        HIPCHK(hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        c->gbatch = batch;
+    }
+    while (true) {
+        if (graph)
+            HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        else
+            for (int i = 0; i < batch; ++i) enqueue_step(c, 1, 1);
+        HIPCHK(hipGetLastError());
+        if ((r = sync_ctl(c))) return r;
+        if (h.done) break;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->n_attempt = h.n_attempt;
+        st->n_accept = h.n_accept;
+        st->t = h.t;
+        st->dt = h.dt;
+        st->ratio = h.diag_valid ? h.ratio : -1;
+        st->energy = h.diag_valid ? h.energy : -1;
+        st->converged = h.converged;
+        st->stopped = h.done && !h.converged;
+        st->nan_seen = h.nan_seen;
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    if (h.nan_seen) return fail(GM_ERR_NAN, "NaN in the error estimate");
+    return GM_OK;
+}
+
+extern "C" int gm_step_log(gm_ctx *c, gm_step_record *out, int n)
+{
+    if (!c || !out) return fail(GM_ERR_ARG, "gm_step_log: null pointer");
+    int r = sync_ctl(c);
+    if (r) return r;
+    long long total = c->ctlh->n_attempt + (c->ctlh->converged ? 1 : 0);
+    long long cnt = std::min<long long>({(long long)n, total, (long long)c->log_cap});
+    std::vector<gmd::StepRecord> all(c->log_cap);
+    HIPCHK(hipMemcpy(all.data(), c->log, c->log_cap * sizeof(gmd::StepRecord), hipMemcpyDeviceToHost));
+    for (long long i = 0; i < cnt; ++i) {
+        long long idx = (total - cnt + i) % c->log_cap;
+        const gmd::StepRecord &s = all[idx];
+        out[i].t = s.t; out[i].dt_used = s.dt_used; out[i].maxDelta = s.maxDelta; out[i].ratio = s.ratio;
+        out[i].energy = s.energy; out[i].jxb_b2 = s.jxb_b2; out[i].jmax = s.jmax;
+        out[i].accepted = s.accepted; out[i].nan_seen = s.nan_seen;
+    }
+    return (int)cnt;
+}
+
+// ---------------------------------------------------------------- timing
+extern "C" int gm_timing_enable(gm_ctx *c, int on)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    c->timing = on != 0;
+    for (int i = 0; i < 3; ++i) { c->tms[i] = 0; c->tcount[i] = 0; }
+    return GM_OK;
+}
+extern "C" int gm_timing_read(gm_ctx *c, double ms[3], int64_t count[3])
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ev_collect(c);
+    for (int i = 0; i < 3; ++i) {
+        if (ms) ms[i] = c->tms[i];
+        if (count) count[i] = c->tcount[i];
+    }
+    return GM_OK;
+}
+extern "C" double gm_algorithmic_bytes_per_point_stage(int precision)
+{
+    // per step: F64 792 B (stages 0-4: 72+24n, final 192), F32K 480 B
+    // (stages 0-4: 48+12n, final 120); divided over six stages
+    return precision == GM_PREC_F32K ? 80.0 : 132.0;
+}
+extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (tx <= 0 || ty <= 0) {
+        auto_tiles(c, cz > 0 ? cz : 512);
+        return ensure_partials(c);
+    }
+    if ((tx + 4) * (ty + 4) > gmd::fused_max_points())
+        return fail(GM_ERR_ARG, "tile rect (tx+4)(ty+4) exceeds the compiled capacity");
+    c->TX = std::min(tx, c->p.nx);
+    c->TY = std::min(ty, c->p.ny);
+    c->tiles_x = (c->p.nx + c->TX - 1) / c->TX;
+    c->tiles_y = (c->p.ny + c->TY - 1) / c->TY;
+    c->CZ = std::max(1, std::min(cz > 0 ? cz : c->p.nz, c->p.nz));
+    c->nchunks = (c->p.nz + c->CZ - 1) / c->CZ;
+    c->nblocks = c->tiles_x * c->tiles_y * c->nchunks;
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    return ensure_partials(c);
+}
+extern "C" int gm_get_tile(gm_ctx *c, int *tx, int *ty, int *cz, int *nblocks)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (tx) *tx = c->TX;
+    if (ty) *ty = c->TY;
+    if (cz) *cz = c->CZ;
+    if (nblocks) *nblocks = c->nblocks;
+    return GM_OK;
+}
+
+// writeCudaInfo (src/io.cu:10-82): device properties, same key/value layout,
+// file name kept as the reference's run helpers expect it.
+extern "C" int gm_write_device_info(const char *dir, int active, int precision_bits)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) return fail(GM_ERR_HIP, "hipGetDeviceCount");
+    std::string d = dir && *dir ? dir : ".";
+    if (d.back() != '/') d += '/';
+    std::string fn = d + "cuda.info";
+    FILE *fd = fopen(fn.c_str(), "w");
+    if (!fd) return fail(GM_ERR_IO, "could not open file '" + fn + "'");
+    fprintf(fd, "number of CUDA devices: %i\n\n", ndev);
+    fprintf(fd, "active device: %li\n\n", (long)active);
+    fprintf(fd, "PRECISION = %i\n\n", precision_bits);
+    for (int dev = 0; dev < ndev; ++dev) {
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, dev) != hipSuccess) continue;
+        fprintf(fd, "device number                   %i\n", dev);
+        fprintf(fd, "name                            %s\n", pr.name);
+        fprintf(fd, "gcn arch                        %s\n", pr.gcnArchName);
+        fprintf(fd, "global memory [bytes]           %zu\n", pr.totalGlobalMem);
+        fprintf(fd, "shared memory per block [bytes] %zu\n", pr.sharedMemPerBlock);
+        fprintf(fd, "registers per block             %d\n", pr.regsPerBlock);
+        fprintf(fd, "threads per warp                %d\n", pr.warpSize);
+        fprintf(fd, "max threads per block           %d\n", pr.maxThreadsPerBlock);
+        fprintf(fd, "clock rate [kHz]                %d\n", pr.clockRate);
+        fprintf(fd, "multiprocessors                 %d\n", pr.multiProcessorCount);
+        fprintf(fd, "memory clock rate [kHz]         %d\n", pr.memoryClockRate);
+        fprintf(fd, "memory bus width [bits]         %d\n", pr.memoryBusWidth);
+        fprintf(fd, "l2 cache size [bytes]           %d\n", pr.l2CacheSize);
+        fprintf(fd, "maximum threads per multiproc.  %d\n", pr.maxThreadsPerMultiProcessor);
+        fprintf(fd, "\n");
+    }
+    fclose(fd);
+    return GM_OK;
+}

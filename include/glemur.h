@@ -1,0 +1,258 @@
+/*
+ * glemur.h -- C ABI of the MI355X-native GLEMuR relaxation library
+ * (libglemur.so).  Plain C types only: no HIP or torch types cross it.
+ *
+ * The reference (SimonCan/glemur) is a monolithic CUDA executable; it has no
+ * FFI.  Each entry point below replaces one step of its host call structure
+ *   init -> boundaries -> current -> gradP -> rungeKutta -> diagnostics
+ * and cites the reference function it stands in for.  Host arrays use the
+ * reference's own layout (AoS, padded (nx+2)(ny+2)(nz+2) points, x fastest);
+ * the library keeps structure-of-arrays copies on the device.
+ *
+ * Every function returns GM_OK (0) or a negative GM_ERR_* code; the text of
+ * the last error of the calling thread is available from gm_last_error().
+ * One host thread per context; each context owns its own HIP stream.
+ */
+#ifndef GLEMUR_H
+#define GLEMUR_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GM_OK               0
+#define GM_ERR_ARG         -1
+#define GM_ERR_HIP         -2
+#define GM_ERR_IO          -3
+#define GM_ERR_UNSUPPORTED -4
+#define GM_ERR_NAN         -5
+#define GM_ERR_PARSE       -6
+
+/* Arithmetic of the stage kernels. */
+#define GM_PREC_F64   0   /* FP64 throughout; follows the reference's operation order */
+#define GM_PREC_F32K  1   /* FP32 stages, positions FP32 hi + compensated lo word;
+                             determinant, error norm, energy and dt in FP64 */
+
+/* Parameters: field for field the reference's struct Parameters with
+ * REAL = double (src/global.h:44-108); bool is one byte. */
+typedef struct gm_params {
+    int    nx, ny, nz;
+    double Lx, Ly, Lz;
+    double Ox, Oy, Oz;
+    double dx, dy, dz;
+    double dx1, dy1, dz1;
+    char   bInit[30];
+    char   uInit[30];
+    double ampl;
+    double phi1, phi2;
+    double L1, L2;
+    double ar, az;
+    int    nBlobs;
+    double width;
+    double minor, major;
+    double stretch;
+    double bGround;
+    double blobXc[10], blobYc[10], blobZc[10], blobZl[10], blobTwist[10], blobScale[10];
+    char   initDist[30];
+    int    initDistCode;
+    double initShearA;
+    double initShearB;
+    double initShearK;
+    double pert;
+    double twist;
+    bool   fRestart;
+    int    nCycle;
+    double dt0;
+    double dtMin;
+    double maxError;
+    bool   zUpdate;
+    bool   inertia;
+    bool   pressure;
+    double nu;
+    double beta;
+    bool   jxbAver;
+    double jxbAverWeight;
+    bool   xPeri;
+    bool   yPeri;
+    bool   zPeri;
+    bool   epsilonProf;
+    char   jMethod[30];
+    char   pMethod[30];
+    int    nTs;
+    double dtDump;
+    double dtSave;
+    bool   dumpJ, dumpDetJac, dumpCellVol, dumpConvexity, dumpWedgeMin;
+    bool   redB2, redB2f, redBMax, redJMax, redJxB_B2Max, redEpsilonStar;
+    bool   redErrB_1ez, redErrXb_XbAn, redConvex, redWedgeMin, redU2, redUMax;
+} gm_params;
+
+/* Reduction results (src/global.h:111-124) plus the relaxation criterion
+ * max|JxB|/(|J||B|) (JxB_JBMax), which the reference does not compute. */
+typedef struct gm_reduction {
+    double B2, B2res, BMax, JMax, JxB_B2Max, epsilonStar, errB_1ez, errXb_XbAn;
+    double convex, wedgeMin, U2, UMax;
+    double JxB_JBMax;
+} gm_reduction;
+
+/* What one attempted step did (src/main.cu:309-383). */
+typedef struct gm_step_info {
+    int    accepted;    /* maxDelta < maxError */
+    int    nan_seen;    /* a NaN error term was met (the reference's max skips it) */
+    int    stopped;     /* the loop condition of src/main.cu:259 now fails */
+    int    converged;   /* stop_ratio reached (gm_run only) */
+    double dt_used;     /* dt of this attempt */
+    double maxDelta;    /* grid max |x5 - x4| */
+    double t;           /* time after the step */
+    double dt;          /* dt for the next attempt */
+    int64_t n_attempt, n_accept;
+} gm_step_info;
+
+/* Fields for gm_download / gm_upload_field. */
+#define GM_FIELD_XB      0   /* 3P  positions (current)                    */
+#define GM_FIELD_B0      1   /* 3P  initial field                          */
+#define GM_FIELD_B       2   /* 3P  field B = J.B0/det (inner + ghost faces) */
+#define GM_FIELD_DETJAC  3   /*  P  det of the grid Jacobian               */
+#define GM_FIELD_J       4   /* 3N  current density, inner AoS             */
+#define GM_FIELD_GRADP   5   /* 3N  pressure gradient, inner AoS           */
+#define GM_FIELD_KK      6   /* 18N six stage slopes, inner AoS, back to back */
+#define GM_FIELD_XB_NEW  7   /* 3P  candidate positions of the last attempt */
+#define GM_FIELD_XB_LO   8   /* 3P  compensation word of the positions (F32K; zeros in F64) */
+
+typedef struct gm_ctx gm_ctx;
+
+/* ---- errors ---------------------------------------------------------- */
+const char *gm_last_error(void);
+int gm_version(void);
+
+/* ---- parameters (src/readNamelist.f90) ------------------------------- */
+/* Defaults of src/readNamelist.f90:92-150, literals rounded to single
+ * precision as the Fortran default-real literals are, plus the derived
+ * dx, dx1 (src/readNamelist.f90:185-186). */
+int gm_params_default(gm_params *p);
+/* readnamelist (src/readNamelist.f90:18-264): reads the groups &comp,
+ * &start, &run, &io from a params.in file over the defaults. */
+int gm_params_read(const char *path, gm_params *p);
+/* Recompute dx, dy, dz, dx1, dy1, dz1 and initDistCode from the rest
+ * (src/readNamelist.f90:185-186,257-262). */
+int gm_params_derive(gm_params *p);
+
+/* ---- host initial state (src/init.cu, src/boundaries.cu host twins) -- */
+/* initState + initDistortion + setPeriHost + setGridPeriHost
+ * (src/main.cu:166-169): B0 and xb, 3P doubles each, reference layout. */
+int gm_init_state(const gm_params *p, double *B0, double *xb);
+/* initResiduals (src/init.cu:13-26). */
+double gm_init_residual(const gm_params *p);
+/* Seeded braid: blobs from splitmix64(seed) as the workloads describe
+ * (no reference counterpart; fills bInit="blobs" and the blob arrays). */
+int gm_braid_seeded(gm_params *p, uint64_t seed, int nblobs_min, int nblobs_max,
+                    int random_centres);
+
+/* ---- context (src/memory.cu) ------------------------------------------ */
+/* allocateMemory (src/memory.cu:10-73) + the constant uploads of
+ * src/main.cu:227-256.  precision: GM_PREC_*. */
+int gm_create(const gm_params *p, int precision, gm_ctx **ctx);
+/* freeMemory (src/memory.cu:77-94). */
+int gm_destroy(gm_ctx *ctx);
+/* H2D of B0 and xb plus the device copies of src/main.cu:181-215. */
+int gm_upload(gm_ctx *ctx, const double *B0, const double *xb, double t, double dt);
+/* D2H of one field into reference layout (src/diagnostics.cu:145-152). */
+int gm_download(gm_ctx *ctx, int field, double *host);
+int gm_get_time(gm_ctx *ctx, double *t, double *dt);
+int gm_set_time(gm_ctx *ctx, double t, double dt);
+int gm_synchronize(gm_ctx *ctx);
+
+/* ---- the reference call structure, one kernel per call (FP64) --------- */
+int gm_b_jacb0(gm_ctx *ctx);                 /* B_JacB0,    src/rungeKutta.cu:28   */
+int gm_set_bbound(gm_ctx *ctx);              /* setBbound,  src/boundaries.cu:637  */
+int gm_set_peri(gm_ctx *ctx, int field);     /* setPeri,    src/boundaries.cu:645 (GM_FIELD_B) */
+int gm_set_grid_peri(gm_ctx *ctx);           /* setGridPeri on xb, src/boundaries.cu:653 */
+int gm_current(gm_ctx *ctx);                 /* current,    src/current.cu:1685    */
+int gm_gradp(gm_ctx *ctx);                   /* gradP,      src/gradP.cu:160       */
+/* kk<<<>>> (src/rungeKutta.cu:112) followed by xb <- xb_tmp (src/main.cu:288). */
+int gm_rk_kk(gm_ctx *ctx, int n, double dt);
+/* xNewStar<<<>>> (src/rungeKutta.cu:251) + findDevMax (src/maths.cu:101). */
+int gm_rk_xnewstar(gm_ctx *ctx, double *maxDelta);
+/* xb <- xb_new (src/main.cu:324), done by exchanging buffers. */
+int gm_accept(gm_ctx *ctx);
+/* reductions (src/diagnostics.cu:12-134) over the fields B, J, detJac the
+ * last gm_fields / gm_b_jacb0+gm_set_bbound+gm_current left on the device:
+ * B2, JMax, JxB_B2Max and JxB_JBMax. */
+int gm_reductions(gm_ctx *ctx, gm_reduction *red);
+
+/* ---- fused path ------------------------------------------------------- */
+/* B, detJac, J (and gradP when pressure is on) for the current positions,
+ * by the fused stage kernel (src/main.cu:343-353); any precision. */
+int gm_fields(gm_ctx *ctx);
+/* One attempted step (src/main.cu:259-383) with fused stage kernels and a
+ * device-side step controller; blocks until the step is done. */
+int gm_step(gm_ctx *ctx, gm_step_info *info);
+
+/* Options of gm_run. */
+typedef struct gm_run_opts {
+    int64_t max_attempts;   /* upper bound on attempted steps               */
+    int     n_cycle;        /* stop after this many accepted steps (<=0: params.nCycle) */
+    double  stop_ratio;     /* stop when max|JxB|/(|J||B|) <= stop_ratio (<=0: off) */
+    int     batch;          /* steps enqueued between host checks (>=1)     */
+    int     use_graph;      /* capture one batch in a hipGraph and replay it */
+} gm_run_opts;
+
+typedef struct gm_run_stats {
+    int64_t n_attempt, n_accept;
+    double  t, dt;
+    double  ratio;          /* last evaluated max|JxB|/(|J||B|)              */
+    double  energy;         /* last evaluated B2                             */
+    int     converged, stopped, nan_seen;
+    double  seconds;        /* wall time of the run, host clock              */
+} gm_run_stats;
+
+/* Many attempted steps without host synchronisation inside a batch: the
+ * device controller applies accept/reject and dt, and every kernel of a
+ * finished run returns at once.  Relaxation criterion evaluated at every
+ * attempt from the stage-0 fields. */
+int gm_run(gm_ctx *ctx, const gm_run_opts *opts, gm_run_stats *stats);
+
+/* Per-step record kept by the device controller (attempt order). */
+typedef struct gm_step_record {
+    double t, dt_used, maxDelta, ratio, energy, jxb_b2, jmax;
+    int32_t accepted, nan_seen;
+} gm_step_record;
+/* Copy the last n records (n <= capacity) into out; returns the count. */
+int gm_step_log(gm_ctx *ctx, gm_step_record *out, int n);
+
+/* ---- kernel timing (HIP events on the context stream) ---------------- */
+/* Enable/disable event timing of the fused stage kernels; gm_timing_read
+ * returns the summed milliseconds and launch count per stage kind
+ * (index 0: stages 0-4, 1: final stage, 2: controller). */
+int gm_timing_enable(gm_ctx *ctx, int on);
+int gm_timing_read(gm_ctx *ctx, double ms[3], int64_t count[3]);
+/* Bytes per grid point and stage the fused kernel must move (algorithmic). */
+double gm_algorithmic_bytes_per_point_stage(int precision);
+/* Fused-kernel tile configuration (x, y tile and z chunk); 0 = automatic. */
+int gm_set_tile(gm_ctx *ctx, int tx, int ty, int cz);
+int gm_get_tile(gm_ctx *ctx, int *tx, int *ty, int *cz, int *nblocks);
+
+/* ---- files (src/io.cu) ------------------------------------------------ */
+/* writeB0 (src/io.cu:167-205): data/B0.vtk in dir. */
+int gm_write_b0(const char *dir, const gm_params *p, const double *B0, int precision);
+/* dumpState (src/io.cu:209-386): n = -1 writes save.vtk, else dump<n>.vtk.
+ * Arrays in reference layout; J is inner AoS; NULL skips optional fields. */
+int gm_dump_state(const char *dir, const gm_params *p, double t, double dt, int n,
+                  const double *xb, const double *B, const double *J,
+                  const double *detJac, int precision);
+/* readState (src/io.cu:390-499): xb, t, dt from save.vtk, B0 from B0.vtk. */
+int gm_read_state(const char *dir, const gm_params *p, double *xb, double *B0,
+                  double *t, double *dt, int precision);
+/* writeTs (src/io.cu:542-678); header when t == 0 (echoed to stdout when
+ * echo != 0).  red may be NULL for the header. */
+int gm_write_ts(const char *dir, const gm_params *p, const gm_reduction *red,
+                double t, double dt, int it, double maxDelta, int echo, int jxb_jb_col);
+/* writeCudaInfo (src/io.cu:10-82): device properties into <dir>/cuda.info. */
+int gm_write_device_info(const char *dir, int active_device, int precision_bits);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,191 @@
+"""ctypes access to the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only the tests, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+this package, as the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = C.CDLL(LIB_PATH)
+
+
+class Params(C.Structure):
+    """or_params: the reference's struct Parameters with REAL = double."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+        ("Ox", C.c_double), ("Oy", C.c_double), ("Oz", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("dx1", C.c_double), ("dy1", C.c_double), ("dz1", C.c_double),
+        ("bInit", C.c_char * 30), ("uInit", C.c_char * 30),
+        ("ampl", C.c_double), ("phi1", C.c_double), ("phi2", C.c_double),
+        ("L1", C.c_double), ("L2", C.c_double), ("ar", C.c_double), ("az", C.c_double),
+        ("nBlobs", C.c_int), ("width", C.c_double), ("minor", C.c_double), ("major", C.c_double),
+        ("stretch", C.c_double), ("bGround", C.c_double),
+        ("blobXc", C.c_double * 10), ("blobYc", C.c_double * 10), ("blobZc", C.c_double * 10),
+        ("blobZl", C.c_double * 10), ("blobTwist", C.c_double * 10), ("blobScale", C.c_double * 10),
+        ("initDist", C.c_char * 30), ("initDistCode", C.c_int),
+        ("initShearA", C.c_double), ("initShearB", C.c_double), ("initShearK", C.c_double),
+        ("pert", C.c_double), ("twist", C.c_double), ("fRestart", C.c_bool),
+        ("nCycle", C.c_int), ("dt0", C.c_double), ("dtMin", C.c_double), ("maxError", C.c_double),
+        ("zUpdate", C.c_bool), ("inertia", C.c_bool), ("pressure", C.c_bool),
+        ("nu", C.c_double), ("beta", C.c_double), ("jxbAver", C.c_bool), ("jxbAverWeight", C.c_double),
+        ("xPeri", C.c_bool), ("yPeri", C.c_bool), ("zPeri", C.c_bool), ("epsilonProf", C.c_bool),
+        ("jMethod", C.c_char * 30), ("pMethod", C.c_char * 30),
+        ("nTs", C.c_int), ("dtDump", C.c_double), ("dtSave", C.c_double),
+        ("dumpJ", C.c_bool), ("dumpDetJac", C.c_bool), ("dumpCellVol", C.c_bool),
+        ("dumpConvexity", C.c_bool), ("dumpWedgeMin", C.c_bool),
+        ("redB2", C.c_bool), ("redB2f", C.c_bool), ("redBMax", C.c_bool), ("redJMax", C.c_bool),
+        ("redJxB_B2Max", C.c_bool), ("redEpsilonStar", C.c_bool), ("redErrB_1ez", C.c_bool),
+        ("redErrXb_XbAn", C.c_bool), ("redConvex", C.c_bool), ("redWedgeMin", C.c_bool),
+        ("redU2", C.c_bool), ("redUMax", C.c_bool),
+    ]
+
+    @property
+    def npad(self):
+        return (self.nx + 2) * (self.ny + 2) * (self.nz + 2)
+
+    @property
+    def ninner(self):
+        return self.nx * self.ny * self.nz
+
+
+class State(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in
+                ("B0", "xb", "B", "detJac", "J", "gradP", "kk", "xb_tmp", "xb_new")] + [
+        ("t", C.c_double), ("dt", C.c_double),
+        ("a", C.c_double * 6), ("b", C.c_double * 42), ("c", C.c_double * 6), ("cs", C.c_double * 6)]
+
+
+_D = C.POINTER(C.c_double)
+_PP = C.POINTER(Params)
+_SIGS = {
+    "or_assign_coefficients": (None, [_D, _D, _D, _D]),
+    "or_init_state": (C.c_int, [_PP, _D, _D]),
+    "or_init_residual": (C.c_double, [_PP]),
+    "or_b_jacb0": (None, [_PP, _D, _D, _D, _D]),
+    "or_update_bbound": (None, [_PP, _D, _D, _D, _D]),
+    "or_set_peri": (None, [_PP, _D]),
+    "or_set_grid_peri": (None, [_PP, _D]),
+    "or_jstokes": (None, [_PP, _D, _D, _D]),
+    "or_gradp": (None, [_PP, _D, _D, _D]),
+    "or_kk": (None, [_PP, _D, C.c_int, C.c_double, _D, _D, _D, _D, _D, _D]),
+    "or_xnewstar": (C.c_double, [_PP, _D, _D, _D, _D, _D, C.POINTER(C.c_int)]),
+    "or_b2": (C.c_double, [_PP, _D, _D]),
+    "or_jmax": (C.c_double, [_PP, _D]),
+    "or_jxb_b2_max": (C.c_double, [_PP, _D, _D]),
+    "or_jxb_jb_max": (C.c_double, [_PP, _D, _D]),
+    "or_state_create": (C.c_int, [_PP, _D, _D, C.POINTER(State)]),
+    "or_state_free": (None, [C.POINTER(State)]),
+    "or_attempt": (C.c_double, [_PP, C.POINTER(State), C.POINTER(C.c_int)]),
+    "or_step": (C.c_int, [_PP, C.POINTER(State), _D, _D, C.POINTER(C.c_int)]),
+    "or_fields": (None, [_PP, C.POINTER(State)]),
+    "or_num_threads": (C.c_int, []),
+    "or_set_threads": (None, [C.c_int]),
+}
+for _n, (_r, _a) in _SIGS.items():
+    _f = getattr(_lib, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+lib = _lib
+
+
+def dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def from_product(pp):
+    """Copy a glemur_amd.Params (same layout) into oracle Params."""
+    q = Params()
+    C.memmove(C.byref(q), C.byref(pp), C.sizeof(Params))
+    return q
+
+
+def coefficients():
+    a, b, c, cs = np.zeros(6), np.zeros(42), np.zeros(6), np.zeros(6)
+    _lib.or_assign_coefficients(dp(a), dp(b), dp(c), dp(cs))
+    return a, b, c, cs
+
+
+def init_state(p):
+    B0, xb = np.zeros(3 * p.npad), np.zeros(3 * p.npad)
+    if _lib.or_init_state(C.byref(p), dp(B0), dp(xb)) != 0:
+        raise ValueError("unsupported bInit")
+    return B0, xb
+
+
+class Sim:
+    """Whole-step state of the oracle (src/main.cu:259-384)."""
+
+    def __init__(self, p, B0, xb):
+        self.p = p
+        self.s = State()
+        self._B0 = np.ascontiguousarray(B0, dtype=np.float64)
+        self._xb = np.ascontiguousarray(xb, dtype=np.float64)
+        if _lib.or_state_create(C.byref(p), dp(self._B0), dp(self._xb), C.byref(self.s)) != 0:
+            raise MemoryError("oracle state")
+
+    def __del__(self):
+        try:
+            _lib.or_state_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    def arr(self, name):
+        n = {"B0": 3 * self.p.npad, "xb": 3 * self.p.npad, "B": 3 * self.p.npad, "detJac": self.p.npad,
+             "J": 3 * self.p.ninner, "gradP": 3 * self.p.ninner, "kk": 18 * self.p.ninner,
+             "xb_tmp": 3 * self.p.npad, "xb_new": 3 * self.p.npad}[name]
+        return np.ctypeslib.as_array(getattr(self.s, name), shape=(n,))
+
+    @property
+    def t(self):
+        return self.s.t
+
+    @property
+    def dt(self):
+        return self.s.dt
+
+    def set_dt(self, dt):
+        self.s.dt = dt
+
+    def fields(self):
+        _lib.or_fields(C.byref(self.p), C.byref(self.s))
+
+    def attempt(self):
+        nan = C.c_int(0)
+        return _lib.or_attempt(C.byref(self.p), C.byref(self.s), C.byref(nan))
+
+    def step(self):
+        dt_used, md, nan = C.c_double(), C.c_double(), C.c_int(0)
+        acc = _lib.or_step(C.byref(self.p), C.byref(self.s), C.byref(dt_used), C.byref(md), C.byref(nan))
+        return bool(acc), dt_used.value, md.value, bool(nan.value)
+
+    def gradp(self):
+        _lib.or_gradp(C.byref(self.p), self.s.xb, self.s.detJac, self.s.gradP)
+
+    def b2(self):
+        return _lib.or_b2(C.byref(self.p), self.s.B, self.s.detJac)
+
+    def jmax(self):
+        return _lib.or_jmax(C.byref(self.p), self.s.J)
+
+    def jxb_b2_max(self):
+        return _lib.or_jxb_b2_max(C.byref(self.p), self.s.B, self.s.J)
+
+    def jxb_jb_max(self):
+        return _lib.or_jxb_jb_max(C.byref(self.p), self.s.B, self.s.J)
+
+
+def set_threads(n):
+    _lib.or_set_threads(int(n))
+
+
+def num_threads():
+    return _lib.or_num_threads()

@@ -1,0 +1,918 @@
+/*
+ * GLEMuR CPU oracle -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Each function restates one reference kernel or host routine in plain C on
+ * the reference's AoS padded layout, point by point, in the reference's
+ * operation order.  Citations give the reference file:line followed.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI 3.141592653589793
+
+/* padded point index (i,j,k in 0..n+1) */
+static inline long pid(const or_params *p, long i, long j, long k)
+{
+    return i + (long)(p->nx + 2) * (j + (long)(p->ny + 2) * k);
+}
+/* inner point index (i,j,k in 0..n-1) */
+static inline long iid(const or_params *p, long i, long j, long k)
+{
+    return i + (long)p->nx * (j + (long)p->ny * k);
+}
+static inline long npad(const or_params *p)
+{
+    return (long)(p->nx + 2) * (p->ny + 2) * (p->nz + 2);
+}
+static inline long ninner(const or_params *p)
+{
+    return (long)p->nx * p->ny * p->nz;
+}
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* src/rungeKutta.cu:10-25 */
+void or_assign_coefficients(double a[6], double b[42], double c[6], double cs[6])
+{
+    static const double A[6] = {0, 0.2, 0.3, 0.6, 1, 0.875};
+    const double B[7][6] = {
+        {0, 0, 0, 0, 0, 0},
+        {0.2, 0, 0, 0, 0, 0},
+        {3 / 40., 9 / 40., 0, 0, 0, 0},
+        {0.3, -0.9, 1.2, 0, 0, 0},
+        {-11 / 54., 2.5, -70 / 27., 35 / 27., 0, 0},
+        {1631 / 55296., 175 / 512., 575 / 13824., 44275 / 110592., 253 / 4096., 0},
+        {0, 0, 0, 0, 0, 0}};
+    const double C[6] = {37 / 378., 0, 250 / 621., 125 / 594., 0, 512 / 1771.};
+    const double CS[6] = {2825 / 27648., 0, 18575 / 48384., 13525 / 55296., 277 / 14336., 0.25};
+    memcpy(a, A, sizeof(A));
+    memcpy(b, B, sizeof(B));
+    memcpy(c, C, sizeof(C));
+    memcpy(cs, CS, sizeof(CS));
+}
+
+static int starts(const char *s, const char *w)
+{
+    return strncmp(s, w, strlen(w)) == 0;
+}
+
+/* src/init.cu:13-26 */
+double or_init_residual(const or_params *p)
+{
+    double r = 0;
+    if (starts(p->bInit, "Pontin09 ") || starts(p->bInit, "analytic ") ||
+        starts(p->bInit, "homZ ") || starts(p->bInit, "blobs "))
+        r = 1 * p->Lx * p->Ly * p->Lz * p->ampl;
+    if (starts(p->bInit, "Borromean "))
+        r = 1 * p->Lx * p->Ly * p->Lz * p->bGround;
+    return r;
+}
+
+/* Periodic copies on the host, src/boundaries.cu:492-528. */
+static void set_peri_planes(const or_params *p, double *f)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    if (p->zPeri)
+        for (int j = 0; j < ny + 2; j++)
+            for (int i = 0; i < nx + 2; i++)
+                for (int l = 0; l < 3; l++) {
+                    f[l + 3 * pid(p, i, j, 0)] = f[l + 3 * pid(p, i, j, nz)];
+                    f[l + 3 * pid(p, i, j, nz + 1)] = f[l + 3 * pid(p, i, j, 1)];
+                }
+    if (p->yPeri)
+        for (int k = 0; k < nz + 2; k++)
+            for (int i = 0; i < nx + 2; i++)
+                for (int l = 0; l < 3; l++) {
+                    f[l + 3 * pid(p, i, 0, k)] = f[l + 3 * pid(p, i, ny, k)];
+                    f[l + 3 * pid(p, i, ny + 1, k)] = f[l + 3 * pid(p, i, 1, k)];
+                }
+    if (p->xPeri)
+        for (int k = 0; k < nz + 2; k++)
+            for (int j = 0; j < ny + 2; j++)
+                for (int l = 0; l < 3; l++) {
+                    f[l + 3 * pid(p, 0, j, k)] = f[l + 3 * pid(p, nx, j, k)];
+                    f[l + 3 * pid(p, nx + 1, j, k)] = f[l + 3 * pid(p, 1, j, k)];
+                }
+}
+
+void or_set_peri(const or_params *p, double *field)
+{
+    set_peri_planes(p, field);
+}
+
+/* Periodic grid copies with the +-(L+h) shift, src/boundaries.cu:532-582.
+ * The host twin (src/boundaries.cu:586-634) applies the shift outside its
+ * loops (brace scoping), so at initialisation the reference leaves the
+ * periodic coordinate of the ghost planes unshifted; this restatement applies
+ * the shift as the device kernel does, at initialisation too. */
+void or_set_grid_peri(const or_params *p, double *xb)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    if (p->zPeri)
+        for (int j = 0; j < ny + 2; j++)
+            for (int i = 0; i < nx + 2; i++) {
+                for (int l = 0; l < 2; l++) {
+                    xb[l + 3 * pid(p, i, j, 0)] = xb[l + 3 * pid(p, i, j, nz)];
+                    xb[l + 3 * pid(p, i, j, nz + 1)] = xb[l + 3 * pid(p, i, j, 1)];
+                }
+                xb[2 + 3 * pid(p, i, j, 0)] = xb[2 + 3 * pid(p, i, j, nz)] - p->Lz - p->dz;
+                xb[2 + 3 * pid(p, i, j, nz + 1)] = xb[2 + 3 * pid(p, i, j, 1)] + p->Lz + p->dz;
+            }
+    if (p->yPeri)
+        for (int k = 0; k < nz + 2; k++)
+            for (int i = 0; i < nx + 2; i++) {
+                for (int l = 0; l < 3; l += 2) {
+                    xb[l + 3 * pid(p, i, 0, k)] = xb[l + 3 * pid(p, i, ny, k)];
+                    xb[l + 3 * pid(p, i, ny + 1, k)] = xb[l + 3 * pid(p, i, 1, k)];
+                }
+                xb[1 + 3 * pid(p, i, 0, k)] = xb[1 + 3 * pid(p, i, ny, k)] - p->Ly - p->dy;
+                xb[1 + 3 * pid(p, i, ny + 1, k)] = xb[1 + 3 * pid(p, i, 1, k)] + p->Ly + p->dy;
+            }
+    if (p->xPeri)
+        for (int k = 0; k < nz + 2; k++)
+            for (int j = 0; j < ny + 2; j++) {
+                for (int l = 1; l < 3; l++) {
+                    xb[l + 3 * pid(p, 0, j, k)] = xb[l + 3 * pid(p, nx, j, k)];
+                    xb[l + 3 * pid(p, nx + 1, j, k)] = xb[l + 3 * pid(p, 1, j, k)];
+                }
+                xb[0 + 3 * pid(p, 0, j, k)] = xb[0 + 3 * pid(p, nx, j, k)] - p->Lx - p->dx;
+                xb[0 + 3 * pid(p, nx + 1, j, k)] = xb[0 + 3 * pid(p, 1, j, k)] + p->Lx + p->dx;
+            }
+}
+
+/* src/init.cu:30-226 (the closed-form fields the configurations use) and
+ * src/init.cu:388-419 (initDistortion), then the periodic host copies
+ * (src/main.cu:166-169). */
+int or_init_state(const or_params *p, double *B0, double *xb)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const char *bi = p->bInit;
+    int known = starts(bi, "Pontin09 ") || starts(bi, "analytic ") || starts(bi, "analyticX ") ||
+                starts(bi, "analyticY ") || starts(bi, "homZ ") || starts(bi, "blobs ");
+    if (!known) return -1;
+    for (int k = 0; k < nz + 2; k++) {
+        double z = p->dz * (k - 1) + p->Oz;
+        for (int j = 0; j < ny + 2; j++) {
+            double y = p->dy * (j - 1) + p->Oy;
+            for (int i = 0; i < nx + 2; i++) {
+                double x = p->dx * (i - 1) + p->Ox;
+                double *b = B0 + 3 * pid(p, i, j, k);
+                double *g = xb + 3 * pid(p, i, j, k);
+                if (starts(bi, "Pontin09 ")) {
+                    double e1 = exp(-(x * x + y * y) / (p->ar * p->ar) - (z - p->L1) * (z - p->L1) / (p->az * p->az));
+                    double e2 = exp(-(x * x + y * y) / (p->ar * p->ar) - (z - p->L2) * (z - p->L2) / (p->az * p->az));
+                    b[0] = -2 * p->ampl / (OR_PI * p->ar) * y * (p->phi1 * e1 + p->phi2 * e2);
+                    b[1] = 2 * p->ampl / (OR_PI * p->ar) * x * (p->phi1 * e1 + p->phi2 * e2);
+                    b[2] = p->ampl;
+                }
+                if (starts(bi, "analytic ")) {
+                    double e = exp(-(x * x + y * y) / (p->ar * p->ar) - z * z / (p->az * p->az));
+                    b[0] = 2 * p->ampl * z * e * p->phi1 / (p->az * p->az) * y;
+                    b[1] = -2 * p->ampl * z * e * p->phi1 / (p->az * p->az) * x;
+                    b[2] = p->ampl;
+                }
+                if (starts(bi, "analyticX ")) {
+                    double e = exp(-(z * z + y * y) / (p->ar * p->ar) - x * x / (p->az * p->az));
+                    b[0] = p->ampl;
+                    b[1] = -2 * p->ampl * x * e * p->phi1 / (p->az * p->az) * z;
+                    b[2] = 2 * p->ampl * x * e * p->phi1 / (p->az * p->az) * y;
+                }
+                if (starts(bi, "analyticY ")) {
+                    double e = exp(-(x * x + z * z) / (p->ar * p->ar) - y * y / (p->az * p->az));
+                    b[0] = 2 * p->ampl * y * e * p->phi1 / (p->az * p->az) * z;
+                    b[1] = p->ampl;
+                    b[2] = -2 * p->ampl * y * e * p->phi1 / (p->az * p->az) * x;
+                }
+                if (starts(bi, "homZ ")) {
+                    b[0] = 0.;
+                    b[1] = 0.;
+                    b[2] = p->ampl;
+                }
+                if (starts(bi, "blobs ")) {
+                    b[0] = 0.;
+                    b[1] = 0.;
+                    b[2] = 0.;
+                    for (int m = 0; m < p->nBlobs; m++) {
+                        double dxc = x - p->blobXc[m], dyc = y - p->blobYc[m], dzc = z - p->blobZc[m];
+                        double e = exp((-dxc * dxc - dyc * dyc) / (p->blobScale[m] * p->blobScale[m]) -
+                                       dzc * dzc / (p->blobZl[m] * p->blobZl[m]));
+                        double amp = 2 * p->ampl * p->blobTwist[m] / p->blobScale[m];
+                        b[0] += amp * e * (-dyc);
+                        b[1] += amp * e * dxc;
+                    }
+                    b[2] = p->ampl;
+                }
+                g[0] = x;
+                g[1] = y;
+                g[2] = z;
+            }
+        }
+    }
+    /* initDistortion, src/init.cu:388-419 */
+    if (!starts(p->initDist, "none ")) {
+        for (int k = 0; k < nz + 2; k++) {
+            double z = p->dz * (k - 1) + p->Oz;
+            for (int j = 0; j < ny + 2; j++) {
+                double y = p->dy * (j - 1) + p->Oy;
+                for (int i = 0; i < nx + 2; i++) {
+                    double x = p->dx * (i - 1) + p->Ox;
+                    double *g = xb + 3 * pid(p, i, j, k);
+                    if (starts(p->initDist, "initShearX ")) {
+                        g[1] -= p->initShearA * sin(p->initShearK * 2 * OR_PI * (x + p->Ox - p->dx / 2) / (p->Lx + p->dx)) * z;
+                        double yy = g[1];
+                        g[0] -= p->initShearB * sin(p->initShearK * 2 * OR_PI * (yy + p->Oy - p->dy / 2) / (p->Ly + p->dy)) * z;
+                    }
+                    if (starts(p->initDist, "centerShift ")) {
+                        g[1] += p->initShearA * exp(-p->initShearK * x * x) * (1 - y * y) * (exp(-1 * z + 1) - 5.0804508111226507);
+                    }
+                }
+            }
+        }
+    }
+    set_peri_planes(p, B0);
+    or_set_grid_peri(p, xb);
+    return 0;
+}
+
+/* src/rungeKutta.cu:28-108 */
+void or_b_jacb0(const or_params *p, const double *xb, const double *B0, double *B, double *detJac)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const double *xm = xb + 3 * pid(p, i - 1, j, k), *xp = xb + 3 * pid(p, i + 1, j, k);
+                const double *ym = xb + 3 * pid(p, i, j - 1, k), *yp = xb + 3 * pid(p, i, j + 1, k);
+                const double *zm = xb + 3 * pid(p, i, j, k - 1), *zp = xb + 3 * pid(p, i, j, k + 1);
+                double jac[3][3];
+                for (int l = 0; l < 3; l++) {
+                    jac[l][0] = (xp[l] - xm[l]) * p->dx1 / 2;
+                    jac[l][1] = (yp[l] - ym[l]) * p->dy1 / 2;
+                    jac[l][2] = (zp[l] - zm[l]) * p->dz1 / 2;
+                }
+                double detJac1 = 1. / (jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
+                                       jac[0][2] * jac[1][0] * jac[2][1] - jac[0][0] * jac[1][2] * jac[2][1] -
+                                       jac[0][1] * jac[1][0] * jac[2][2] - jac[0][2] * jac[1][1] * jac[2][0]);
+                long c = pid(p, i, j, k);
+                const double *b0 = B0 + 3 * c;
+                for (int l = 0; l < 3; l++)
+                    B[l + 3 * c] = (jac[l][0] * b0[0] + jac[l][1] * b0[1] + jac[l][2] * b0[2]) * detJac1;
+                detJac[c] = 1. / detJac1;
+            }
+}
+
+/* addDist, src/boundaries.cu:11-35 (x = dx*i + Ox from the inner index). */
+static double add_dist(const or_params *p, int i, int j, double z, int l)
+{
+    double x = p->dx * i + p->Ox;
+    double y = p->dy * j + p->Oy;
+    double distX = 0, distY = 0;
+    if (p->initDistCode == 0) {
+        distY = -p->initShearA * sin(p->initShearK * 2 * OR_PI * (x + p->Ox - p->dx / 2) / (p->Lx + p->dx)) * z;
+        distX = -p->initShearB * sin(p->initShearK * 2 * OR_PI * (distY + y + p->Oy - p->dy / 2) / (p->Ly + p->dy)) * z;
+    }
+    if (p->initDistCode == 1) {
+        distX = 0;
+        distY = p->initShearA * exp(-p->initShearK * x * x) * (1 - y * y) * (exp(-1 * z + 1) - 0.15366);
+    }
+    return l == 0 ? distX : distY;
+}
+
+/* jac, 1/det and B with the normal component held (normal < 0: none). */
+static void bound_b(const or_params *p, double jac[3][3], const double *b0, int normal,
+                    double *Bout, double *detOut)
+{
+    double detJac1 = 1 / (jac[0][0] * jac[1][1] * jac[2][2] + jac[0][1] * jac[1][2] * jac[2][0] +
+                          jac[0][2] * jac[1][0] * jac[2][1] - jac[0][0] * jac[1][2] * jac[2][1] -
+                          jac[0][1] * jac[1][0] * jac[2][2] - jac[0][2] * jac[1][1] * jac[2][0]);
+    (void)p;
+    for (int l = 0; l < 3; l++) {
+        if (l == normal)
+            Bout[l] = b0[l];
+        else
+            Bout[l] = (jac[l][0] * b0[0] + jac[l][1] * b0[1] + jac[l][2] * b0[2]) * detJac1;
+    }
+    *detOut = 1 / detJac1;
+}
+
+
+/* Periodic image coordinate as the reference forms it: a value read by a
+ * neighbouring thread of the launch tile is shifted as (a -+ L) -+ h, a value
+ * read at a tile edge as a + (-+(L + h)) (src/boundaries.cu:112-160). */
+static double pshift(double a, int side, double L, double h, int edge)
+{
+    if (edge) return a + (2 * side - 1) * (L + h);
+    return side ? a + L + h : a - L - h;
+}
+
+/* updateBbound, src/boundaries.cu:39-446, faces 2 (z), 1 (y), 0 (x).
+ * Line-tied faces: the outer ghost is the ghost point moved one cell along
+ * the outward normal (plus addDist on z faces); in-plane neighbours at the
+ * edges of the reference's 16x16 launch tiles and at the last row are
+ * rebuilt as x +- h instead of being read (src/boundaries.cu:72-109). */
+void or_update_bbound(const or_params *p, const double *xb, const double *B0, double *B, double *detJac)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const double dx = p->dx, dy = p->dy, dz = p->dz;
+
+    /* ---- face 2: z = const planes ---- */
+    for (int side = 0; side < 2; side++) {
+        int kg = side ? nz + 1 : 0;
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                double L0[3], L2[3], xl[3], xr[3], yd[3], yu[3], jac[3][3];
+                long g = pid(p, i + 1, j + 1, kg);
+                const double *b0;
+                if (!p->zPeri) {
+                    const double *xg = xb + 3 * g;
+                    const double *xin = xb + 3 * pid(p, i + 1, j + 1, side ? nz : 1);
+                    double x2[3];
+                    double zsec = 2 * dz * (2 * side - 1) + p->Oz + p->Lz * side;
+                    double zg = dz * (2 * side - 1) + p->Oz + p->Lz * side;
+                    for (int l = 0; l < 2; l++)
+                        x2[l] = xg[l] - add_dist(p, i, j, xg[2], l) + add_dist(p, i, j, zsec, l);
+                    x2[2] = xg[2] + (2 * side - 1) * dz;
+                    for (int l = 0; l < 3; l++) {
+                        L0[l] = side ? xin[l] : x2[l];
+                        L2[l] = side ? x2[l] : xin[l];
+                    }
+                    if (i % 16 == 0) {
+                        xl[0] = xg[0] - dx - add_dist(p, i, j, zg, 0) + add_dist(p, i - 1, j, zg, 0);
+                        xl[1] = xg[1] - add_dist(p, i, j, zg, 1) + add_dist(p, i - 1, j, zg, 1);
+                        xl[2] = xg[2];
+                    } else
+                        memcpy(xl, xb + 3 * pid(p, i, j + 1, kg), sizeof xl);
+                    if (i % 16 == 15 || i == nx - 1) {
+                        xr[0] = xg[0] + dx - add_dist(p, i, j, zg, 0) + add_dist(p, i + 1, j, zg, 0);
+                        xr[1] = xg[1] - add_dist(p, i, j, zg, 1) + add_dist(p, i + 1, j, zg, 1);
+                        xr[2] = xg[2];
+                    } else
+                        memcpy(xr, xb + 3 * pid(p, i + 2, j + 1, kg), sizeof xr);
+                    if (j % 16 == 0) {
+                        yd[0] = xg[0] - add_dist(p, i, j, zg, 0) + add_dist(p, i, j - 1, zg, 0);
+                        yd[1] = xg[1] - dy - add_dist(p, i, j, zg, 1) + add_dist(p, i, j - 1, zg, 1);
+                        yd[2] = xg[2];
+                    } else
+                        memcpy(yd, xb + 3 * pid(p, i + 1, j, kg), sizeof yd);
+                    if (j % 16 == 15 || j == ny - 1) {
+                        yu[0] = xg[0] - add_dist(p, i, j, zg, 0) + add_dist(p, i, j + 1, zg, 0);
+                        yu[1] = xg[1] + dy - add_dist(p, i, j, zg, 1) + add_dist(p, i, j + 1, zg, 1);
+                        yu[2] = xg[2];
+                    } else
+                        memcpy(yu, xb + 3 * pid(p, i + 1, j + 2, kg), sizeof yu);
+                    b0 = B0 + 3 * g;
+                } else {
+                    /* periodic: the ghost is evaluated at the image plane
+                     * with the shifted coordinate (src/boundaries.cu:112-162) */
+                    int k0 = side ? nz : nz - 1, k2 = side ? 2 : 1;
+                    int ke = side ? 1 : nz;          /* image plane of the ghost */
+                    const double *a0 = xb + 3 * pid(p, i + 1, j + 1, k0);
+                    const double *a2 = xb + 3 * pid(p, i + 1, j + 1, k2);
+                    for (int l = 0; l < 2; l++) {
+                        L0[l] = a0[l];
+                        L2[l] = a2[l];
+                    }
+                    L0[2] = side ? a0[2] : a0[2] - p->Lz - p->dz;
+                    L2[2] = side ? a2[2] + p->Lz + p->dz : a2[2];
+                    const double *ql = xb + 3 * pid(p, i, j + 1, ke), *qr = xb + 3 * pid(p, i + 2, j + 1, ke);
+                    const double *qd = xb + 3 * pid(p, i + 1, j, ke), *qu = xb + 3 * pid(p, i + 1, j + 2, ke);
+                    for (int l = 0; l < 2; l++) {
+                        xl[l] = ql[l];
+                        xr[l] = qr[l];
+                        yd[l] = qd[l];
+                        yu[l] = qu[l];
+                    }
+                    xl[2] = pshift(ql[2], side, p->Lz, p->dz, i % 16 == 0);
+                    xr[2] = pshift(qr[2], side, p->Lz, p->dz, i % 16 == 15 || i == nx - 1);
+                    yd[2] = pshift(qd[2], side, p->Lz, p->dz, j % 16 == 0);
+                    yu[2] = pshift(qu[2], side, p->Lz, p->dz, j % 16 == 15 || j == ny - 1);
+                    b0 = B0 + 3 * pid(p, i + 1, j + 1, ke);
+                }
+                for (int l = 0; l < 3; l++) {
+                    jac[l][0] = (xr[l] - xl[l]) * p->dx1 / 2;
+                    jac[l][1] = (yu[l] - yd[l]) * p->dy1 / 2;
+                    jac[l][2] = (L2[l] - L0[l]) * p->dz1 / 2;
+                }
+                bound_b(p, jac, b0, p->zPeri ? -1 : 2, B + 3 * g, detJac + g);
+            }
+    }
+
+    /* ---- face 1: y = const planes ---- */
+    for (int side = 0; side < 2; side++) {
+        int jg = side ? ny + 1 : 0;
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nz; k++)
+            for (int i = 0; i < nx; i++) {
+                double L0[3], L2[3], xl[3], xr[3], zd[3], zu[3], jac[3][3];
+                long g = pid(p, i + 1, jg, k + 1);
+                const double *b0;
+                if (!p->yPeri) {
+                    const double *xg = xb + 3 * g;
+                    const double *xin = xb + 3 * pid(p, i + 1, side ? ny : 1, k + 1);
+                    double x2[3] = {xg[0], xg[1] + (2 * side - 1) * dy, xg[2]};
+                    for (int l = 0; l < 3; l++) {
+                        L0[l] = side ? xin[l] : x2[l];
+                        L2[l] = side ? x2[l] : xin[l];
+                    }
+                    if (i % 16 == 0) {
+                        xl[0] = xg[0] - dx; xl[1] = xg[1]; xl[2] = xg[2];
+                    } else
+                        memcpy(xl, xb + 3 * pid(p, i, jg, k + 1), sizeof xl);
+                    if (i % 16 == 15 || i == nx - 1) {
+                        xr[0] = xg[0] + dx; xr[1] = xg[1]; xr[2] = xg[2];
+                    } else
+                        memcpy(xr, xb + 3 * pid(p, i + 2, jg, k + 1), sizeof xr);
+                    if (k % 16 == 0) {
+                        zd[0] = xg[0]; zd[1] = xg[1]; zd[2] = xg[2] - dz;
+                    } else
+                        memcpy(zd, xb + 3 * pid(p, i + 1, jg, k), sizeof zd);
+                    if (k % 16 == 15 || k == nz - 1) {
+                        zu[0] = xg[0]; zu[1] = xg[1]; zu[2] = xg[2] + dz;
+                    } else
+                        memcpy(zu, xb + 3 * pid(p, i + 1, jg, k + 2), sizeof zu);
+                    b0 = B0 + 3 * g;
+                } else {
+                    int j0 = side ? ny : ny - 1, j2 = side ? 2 : 1;
+                    int je = side ? 1 : ny;
+                    const double *a0 = xb + 3 * pid(p, i + 1, j0, k + 1);
+                    const double *a2 = xb + 3 * pid(p, i + 1, j2, k + 1);
+                    for (int l = 0; l < 3; l += 2) {
+                        L0[l] = a0[l];
+                        L2[l] = a2[l];
+                    }
+                    L0[1] = side ? a0[1] : a0[1] - p->Ly - p->dy;
+                    L2[1] = side ? a2[1] + p->Ly + p->dy : a2[1];
+                    const double *ql = xb + 3 * pid(p, i, je, k + 1), *qr = xb + 3 * pid(p, i + 2, je, k + 1);
+                    const double *qd = xb + 3 * pid(p, i + 1, je, k), *qu = xb + 3 * pid(p, i + 1, je, k + 2);
+                    for (int l = 0; l < 3; l += 2) {
+                        xl[l] = ql[l];
+                        xr[l] = qr[l];
+                        zd[l] = qd[l];
+                        zu[l] = qu[l];
+                    }
+                    xl[1] = pshift(ql[1], side, p->Ly, p->dy, i % 16 == 0);
+                    xr[1] = pshift(qr[1], side, p->Ly, p->dy, i % 16 == 15 || i == nx - 1);
+                    zd[1] = pshift(qd[1], side, p->Ly, p->dy, k % 16 == 0);
+                    zu[1] = pshift(qu[1], side, p->Ly, p->dy, k % 16 == 15 || k == nz - 1);
+                    b0 = B0 + 3 * pid(p, i + 1, je, k + 1);
+                }
+                for (int l = 0; l < 3; l++) {
+                    jac[l][0] = (xr[l] - xl[l]) * p->dx1 / 2;
+                    jac[l][1] = (L2[l] - L0[l]) * p->dy1 / 2;
+                    jac[l][2] = (zu[l] - zd[l]) * p->dz1 / 2;
+                }
+                bound_b(p, jac, b0, p->yPeri ? -1 : 1, B + 3 * g, detJac + g);
+            }
+    }
+
+    /* ---- face 0: x = const planes ---- */
+    for (int side = 0; side < 2; side++) {
+        int ig = side ? nx + 1 : 0;
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++) {
+                double L0[3], L2[3], yl[3], yr[3], zd[3], zu[3], jac[3][3];
+                long g = pid(p, ig, j + 1, k + 1);
+                const double *b0;
+                if (!p->xPeri) {
+                    const double *xg = xb + 3 * g;
+                    const double *xin = xb + 3 * pid(p, side ? nx : 1, j + 1, k + 1);
+                    double x2[3] = {xg[0] + (2 * side - 1) * dx, xg[1], xg[2]};
+                    for (int l = 0; l < 3; l++) {
+                        L0[l] = side ? xin[l] : x2[l];
+                        L2[l] = side ? x2[l] : xin[l];
+                    }
+                    if (j % 16 == 0) {
+                        yl[0] = xg[0]; yl[1] = xg[1] - dy; yl[2] = xg[2];
+                    } else
+                        memcpy(yl, xb + 3 * pid(p, ig, j, k + 1), sizeof yl);
+                    if (j % 16 == 15 || j == ny - 1) {
+                        yr[0] = xg[0]; yr[1] = xg[1] + dy; yr[2] = xg[2];
+                    } else
+                        memcpy(yr, xb + 3 * pid(p, ig, j + 2, k + 1), sizeof yr);
+                    if (k % 16 == 0) {
+                        zd[0] = xg[0]; zd[1] = xg[1]; zd[2] = xg[2] - dz;
+                    } else
+                        memcpy(zd, xb + 3 * pid(p, ig, j + 1, k), sizeof zd);
+                    if (k % 16 == 15 || k == nz - 1) {
+                        zu[0] = xg[0]; zu[1] = xg[1]; zu[2] = xg[2] + dz;
+                    } else
+                        memcpy(zu, xb + 3 * pid(p, ig, j + 1, k + 2), sizeof zu);
+                    b0 = B0 + 3 * g;
+                } else {
+                    int i0 = side ? nx : nx - 1, i2 = side ? 2 : 1;
+                    int ie = side ? 1 : nx;
+                    const double *a0 = xb + 3 * pid(p, i0, j + 1, k + 1);
+                    const double *a2 = xb + 3 * pid(p, i2, j + 1, k + 1);
+                    for (int l = 1; l < 3; l++) {
+                        L0[l] = a0[l];
+                        L2[l] = a2[l];
+                    }
+                    L0[0] = side ? a0[0] : a0[0] - p->Lx - p->dx;
+                    L2[0] = side ? a2[0] + p->Lx + p->dx : a2[0];
+                    const double *ql = xb + 3 * pid(p, ie, j, k + 1), *qr = xb + 3 * pid(p, ie, j + 2, k + 1);
+                    const double *qd = xb + 3 * pid(p, ie, j + 1, k), *qu = xb + 3 * pid(p, ie, j + 1, k + 2);
+                    for (int l = 1; l < 3; l++) {
+                        yl[l] = ql[l];
+                        yr[l] = qr[l];
+                        zd[l] = qd[l];
+                        zu[l] = qu[l];
+                    }
+                    yl[0] = pshift(ql[0], side, p->Lx, p->dx, j % 16 == 0);
+                    yr[0] = pshift(qr[0], side, p->Lx, p->dx, j % 16 == 15 || j == ny - 1);
+                    zd[0] = pshift(qd[0], side, p->Lx, p->dx, k % 16 == 0);
+                    zu[0] = pshift(qu[0], side, p->Lx, p->dx, k % 16 == 15 || k == nz - 1);
+                    b0 = B0 + 3 * pid(p, ie, j + 1, k + 1);
+                }
+                for (int l = 0; l < 3; l++) {
+                    jac[l][0] = (L2[l] - L0[l]) * p->dx1 / 2;
+                    jac[l][1] = (yr[l] - yl[l]) * p->dy1 / 2;
+                    jac[l][2] = (zu[l] - zd[l]) * p->dz1 / 2;
+                }
+                bound_b(p, jac, b0, p->xPeri ? -1 : 0, B + 3 * g, detJac + g);
+            }
+    }
+}
+
+static inline void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* One diamond of the Stokes current (src/current.cu:291-320): vertices in
+ * circulation order v1..v4; returns the circulation and the mean area vector. */
+static double diamond(const double *x[4], const double *b[4], double na[3])
+{
+    double d[4][3], c[4][3];
+    double I = 0;
+    for (int l = 0; l < 3; l++) {
+        d[0][l] = x[1][l] - x[0][l];
+        d[1][l] = x[2][l] - x[1][l];
+        d[2][l] = x[3][l] - x[2][l];
+        d[3][l] = x[0][l] - x[3][l];
+        I += (b[1][l] + b[0][l]) / 2 * d[0][l] + (b[2][l] + b[1][l]) / 2 * d[1][l] +
+             (b[3][l] + b[2][l]) / 2 * d[2][l] + (b[0][l] + b[3][l]) / 2 * d[3][l];
+    }
+    cross3(d[0], d[1], c[0]);
+    cross3(d[1], d[2], c[1]);
+    cross3(d[2], d[3], c[2]);
+    cross3(d[3], d[0], c[3]);
+    for (int l = 0; l < 3; l++)
+        na[l] = (c[0][l] + c[1][l] + c[2][l] + c[3][l]) / 4;
+    return I;
+}
+
+/* src/current.cu:222-409 */
+void or_jstokes(const or_params *p, const double *xb, const double *B, double *J)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                long im = pid(p, i - 1, j, k), ip = pid(p, i + 1, j, k);
+                long jm = pid(p, i, j - 1, k), jp = pid(p, i, j + 1, k);
+                long km = pid(p, i, j, k - 1), kp = pid(p, i, j, k + 1);
+                double NA[3][3], II[3];
+                {   /* quadrilateral 0: W, S, E, N */
+                    const double *x[4] = {xb + 3 * im, xb + 3 * jm, xb + 3 * ip, xb + 3 * jp};
+                    const double *b[4] = {B + 3 * im, B + 3 * jm, B + 3 * ip, B + 3 * jp};
+                    II[0] = diamond(x, b, NA[0]);
+                }
+                {   /* quadrilateral 1: y-, z-, y+, z+ */
+                    const double *x[4] = {xb + 3 * jm, xb + 3 * km, xb + 3 * jp, xb + 3 * kp};
+                    const double *b[4] = {B + 3 * jm, B + 3 * km, B + 3 * jp, B + 3 * kp};
+                    II[1] = diamond(x, b, NA[1]);
+                }
+                {   /* quadrilateral 2: z-, x-, z+, x+ */
+                    const double *x[4] = {xb + 3 * km, xb + 3 * im, xb + 3 * kp, xb + 3 * ip};
+                    const double *b[4] = {B + 3 * km, B + 3 * im, B + 3 * kp, B + 3 * ip};
+                    II[2] = diamond(x, b, NA[2]);
+                }
+                double detN = NA[0][0] * NA[1][1] * NA[2][2] + NA[0][1] * NA[1][2] * NA[2][0] +
+                              NA[0][2] * NA[1][0] * NA[2][1] - NA[0][0] * NA[1][2] * NA[2][1] -
+                              NA[0][1] * NA[1][0] * NA[2][2] - NA[0][2] * NA[1][1] * NA[2][0];
+                double detN1 = 1 / detN;
+                double A1[3][3];
+                A1[0][0] = (NA[1][1] * NA[2][2] - NA[1][2] * NA[2][1]) * detN1;
+                A1[0][1] = (NA[0][2] * NA[2][1] - NA[0][1] * NA[2][2]) * detN1;
+                A1[0][2] = (NA[0][1] * NA[1][2] - NA[0][2] * NA[1][1]) * detN1;
+                A1[1][0] = (NA[1][2] * NA[2][0] - NA[1][0] * NA[2][2]) * detN1;
+                A1[1][1] = (NA[0][0] * NA[2][2] - NA[0][2] * NA[2][0]) * detN1;
+                A1[1][2] = (NA[0][2] * NA[1][0] - NA[0][0] * NA[1][2]) * detN1;
+                A1[2][0] = (NA[1][0] * NA[2][1] - NA[1][1] * NA[2][0]) * detN1;
+                A1[2][1] = (NA[0][1] * NA[2][0] - NA[0][0] * NA[2][1]) * detN1;
+                A1[2][2] = (NA[0][0] * NA[1][1] - NA[0][1] * NA[1][0]) * detN1;
+                double *Jo = J + 3 * iid(p, i - 1, j - 1, k - 1);
+                for (int l = 0; l < 3; l++) {
+                    double s = 0;
+                    for (int m = 0; m < 3; m++)
+                        s += A1[l][m] * II[m];
+                    Jo[l] = s;
+                }
+            }
+}
+
+/* src/gradP.cu:10-156 */
+void or_gradp(const or_params *p, const double *xb, const double *detJac, double *gradP)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+#define X(di, dj, dk, l) xb[(l) + 3 * pid(p, i + (di), j + (dj), k + (dk))]
+                double det1 = 1 / detJac[pid(p, i, j, k)];
+                double jac[3][3], jac1[3][3], d2[3][3][3];
+                for (int l = 0; l < 3; l++) {
+                    jac[l][0] = (X(1, 0, 0, l) - X(-1, 0, 0, l)) * p->dx1 / 2;
+                    jac[l][1] = (X(0, 1, 0, l) - X(0, -1, 0, l)) * p->dy1 / 2;
+                    jac[l][2] = (X(0, 0, 1, l) - X(0, 0, -1, l)) * p->dz1 / 2;
+                }
+                jac1[0][0] = (jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1]) * det1;
+                jac1[0][1] = (jac[0][2] * jac[2][1] - jac[0][1] * jac[2][2]) * det1;
+                jac1[0][2] = (jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1]) * det1;
+                jac1[1][0] = (jac[1][2] * jac[2][0] - jac[1][0] * jac[2][2]) * det1;
+                jac1[1][1] = (jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0]) * det1;
+                jac1[1][2] = (jac[0][2] * jac[1][0] - jac[0][0] * jac[1][2]) * det1;
+                jac1[2][0] = (jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0]) * det1;
+                jac1[2][1] = (jac[0][1] * jac[2][0] - jac[0][0] * jac[2][1]) * det1;
+                jac1[2][2] = (jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0]) * det1;
+                for (int l = 0; l < 3; l++) {
+                    d2[l][0][0] = (X(1, 0, 0, l) - 2 * X(0, 0, 0, l) + X(-1, 0, 0, l)) * p->dx1 * p->dx1;
+                    d2[l][1][1] = (X(0, 1, 0, l) - 2 * X(0, 0, 0, l) + X(0, -1, 0, l)) * p->dy1 * p->dy1;
+                    d2[l][2][2] = (X(0, 0, 1, l) - 2 * X(0, 0, 0, l) + X(0, 0, -1, l)) * p->dz1 * p->dz1;
+                    d2[l][0][1] = (X(1, 1, 0, l) - X(1, -1, 0, l) - X(-1, 1, 0, l) + X(-1, -1, 0, l)) * p->dx1 * p->dy1 / 4;
+                    d2[l][0][2] = (X(1, 0, 1, l) - X(1, 0, -1, l) - X(-1, 0, 1, l) + X(-1, 0, -1, l)) * p->dx1 * p->dz1 / 4;
+                    d2[l][1][2] = (X(0, 1, 1, l) - X(0, -1, 1, l) - X(0, 1, -1, l) + X(0, -1, -1, l)) * p->dy1 * p->dz1 / 4;
+                    d2[l][1][0] = d2[l][0][1];
+                    d2[l][2][0] = d2[l][0][2];
+                    d2[l][2][1] = d2[l][1][2];
+                }
+#undef X
+                double g[3] = {0, 0, 0};
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        for (int l = 0; l < 3; l++) {
+                            g[0] -= d2[a][b][l] * jac1[b][a] * jac1[l][0];
+                            g[1] -= d2[a][b][l] * jac1[b][a] * jac1[l][1];
+                            g[2] -= d2[a][b][l] * jac1[b][a] * jac1[l][2];
+                        }
+                double *go = gradP + 3 * iid(p, i - 1, j - 1, k - 1);
+                for (int l = 0; l < 3; l++)
+                    go[l] = g[l] * (p->beta * det1);
+            }
+}
+
+/* epsilon profile of src/rungeKutta.cu:180-185 (inner indices). */
+static double eps_prof(const or_params *p, int i, int j, int k)
+{
+    if (!p->epsilonProf) return 1;
+    return (-(i - (p->nx) / 2.) / (p->nx) * (i + 1 - (p->nx) / 2.) / (p->nx) * 4 + 1) *
+           (-(j - (p->ny) / 2.) / (p->ny) * (j + 1 - (p->ny) / 2.) / (p->ny) * 4 + 1) *
+           (-(k - (p->nz) / 2.) / (p->nz) * (k + 1 - (p->nz) / 2.) / (p->nz) * 4 + 1);
+}
+
+/* src/rungeKutta.cu:112-247, magnetofrictional branch (inertia and jxbAver
+ * off). */
+void or_kk(const or_params *p, const double b[42], int n, double dt, const double *xb, const double *B,
+           const double *J, const double *gradP, double *kk, double *xb_tmp)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const long N = ninner(p);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                long c = pid(p, i + 1, j + 1, k + 1), q = iid(p, i, j, k);
+                double kv[3];
+                cross3(J + 3 * q, B + 3 * c, kv);
+                double eps = eps_prof(p, i, j, k);
+                for (int l = 0; l < 3; l++) {
+                    kv[l] *= dt * eps;
+                    if (p->pressure) kv[l] += -dt * gradP[l + 3 * q];
+                }
+                for (int l = 0; l < 3; l++)
+                    kk[l + 3 * q + 3 * N * n] = kv[l];
+                for (int l = 0; l < 3; l++) {
+                    double x = xb[l + 3 * c];
+                    for (int m = 0; m < n + 1; m++)
+                        x += (b[m + (n + 1) * 6] - b[m + n * 6]) * kk[l + 3 * q + 3 * N * m];
+                    xb_tmp[l + 3 * c] = x;
+                }
+            }
+}
+
+/* src/rungeKutta.cu:251-353 with the block max of src/maths.cu:144-153 and
+ * the grid max of src/maths.cu:58-130 (both skip NaN: they start at 0 and
+ * keep a value only when '>' holds). */
+double or_xnewstar(const or_params *p, const double c[6], const double cs[6], const double *xb,
+                   const double *kk, double *xb_new, int *nan_seen)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const long N = ninner(p);
+    double mx = 0;
+    int nan = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                long g = pid(p, i + 1, j + 1, k + 1), q = iid(p, i, j, k);
+                for (int l = 0; l < 3; l++) {
+                    double xn = xb[l + 3 * g], xs = xb[l + 3 * g];
+                    if (l < 2 || p->zUpdate)
+                        for (int m = 0; m < 6; m++) {
+                            xn += c[m] * kk[l + 3 * q + 3 * N * m];
+                            xs += cs[m] * kk[l + 3 * q + 3 * N * m];
+                        }
+                    xb_new[l + 3 * g] = xn;
+                    double d = fabs(xn - xs);
+                    if (d != d) nan = 1;
+                    if (d > mx) mx = d;
+                }
+            }
+    if (nan_seen) *nan_seen = nan;
+    return mx;
+}
+
+/* B2_det + sumGlobal, src/maths.cu:647-680, src/diagnostics.cu:73-82.
+ * Summed serially in (k,j,i) order. */
+double or_b2(const or_params *p, const double *B, const double *detJac)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double s = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                long g = pid(p, i + 1, j + 1, k + 1);
+                const double *b = B + 3 * g;
+                double v = (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) * detJac[g];
+                int bound = (i == 0) + (j == 0) + (k == 0) + (i == nx - 1) + (j == ny - 1) + (k == nz - 1);
+                bound = 1 << bound;
+                v /= bound;
+                s += v;
+            }
+    return s * p->dx * p->dy * p->dz;
+}
+
+/* mag + findDevMax, src/maths.cu:157-164, src/diagnostics.cu:15-22 */
+double or_jmax(const or_params *p, const double *J)
+{
+    const long N = ninner(p);
+    double mx = 0;
+    for (long q = 0; q < N; q++) {
+        const double *j = J + 3 * q;
+        double v = sqrt(j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+        if (v > mx) mx = v;
+    }
+    return mx;
+}
+
+/* JxB_B2 + findDevMax, src/maths.cu:168-200, src/diagnostics.cu:23-32 */
+double or_jxb_b2_max(const or_params *p, const double *B, const double *J)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double mx = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double *b = B + 3 * pid(p, i + 1, j + 1, k + 1);
+                const double *jj = J + 3 * iid(p, i, j, k);
+                double c[3];
+                cross3(jj, b, c);
+                double b2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+                double v = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) / b2;
+                if (v > mx) mx = v;
+            }
+    return mx;
+}
+
+/* max |JxB|/(|J||B|), 0/0 := 0.  No reference counterpart: the relaxation
+ * criterion of the workload. */
+double or_jxb_jb_max(const or_params *p, const double *B, const double *J)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double mx = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double *b = B + 3 * pid(p, i + 1, j + 1, k + 1);
+                const double *jj = J + 3 * iid(p, i, j, k);
+                double c[3];
+                cross3(jj, b, c);
+                double nc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                double nj = sqrt(jj[0] * jj[0] + jj[1] * jj[1] + jj[2] * jj[2]);
+                double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+                double den = nj * nb;
+                double v = den > 0 ? nc / den : 0;
+                if (v > mx) mx = v;
+            }
+    return mx;
+}
+
+int or_state_create(const or_params *p, const double *B0, const double *xb, or_state *s)
+{
+    long P3 = 3 * npad(p), N3 = 3 * ninner(p);
+    memset(s, 0, sizeof *s);
+    s->B0 = (double *)malloc(P3 * sizeof(double));
+    s->xb = (double *)malloc(P3 * sizeof(double));
+    s->B = (double *)malloc(P3 * sizeof(double));
+    s->detJac = (double *)calloc(npad(p), sizeof(double));
+    s->J = (double *)calloc(N3, sizeof(double));
+    s->gradP = (double *)calloc(N3, sizeof(double));
+    s->kk = (double *)calloc(6 * N3, sizeof(double));
+    s->xb_tmp = (double *)malloc(P3 * sizeof(double));
+    s->xb_new = (double *)malloc(P3 * sizeof(double));
+    if (!s->B0 || !s->xb || !s->B || !s->detJac || !s->J || !s->gradP || !s->kk || !s->xb_tmp || !s->xb_new) {
+        or_state_free(s);
+        return -1;
+    }
+    memcpy(s->B0, B0, P3 * sizeof(double));
+    memcpy(s->xb, xb, P3 * sizeof(double));
+    memcpy(s->B, B0, P3 * sizeof(double));
+    memcpy(s->xb_tmp, xb, P3 * sizeof(double));
+    memcpy(s->xb_new, xb, P3 * sizeof(double));
+    s->t = 0;
+    s->dt = p->dt0;
+    or_assign_coefficients(s->a, s->b, s->c, s->cs);
+    return 0;
+}
+
+void or_state_free(or_state *s)
+{
+    free(s->B0); free(s->xb); free(s->B); free(s->detJac); free(s->J);
+    free(s->gradP); free(s->kk); free(s->xb_tmp); free(s->xb_new);
+    memset(s, 0, sizeof *s);
+}
+
+static int any_peri(const or_params *p)
+{
+    return p->xPeri || p->yPeri || p->zPeri;
+}
+
+void or_fields(const or_params *p, or_state *s)
+{
+    or_b_jacb0(p, s->xb, s->B0, s->B, s->detJac);
+    or_update_bbound(p, s->xb, s->B0, s->B, s->detJac);
+    if (any_peri(p)) or_set_peri(p, s->B);
+    or_jstokes(p, s->xb, s->B, s->J);
+}
+
+/* src/main.cu:260-319 */
+double or_attempt(const or_params *p, or_state *s, int *nan_seen)
+{
+    long P3 = 3 * npad(p);
+    for (int n = 0; n < 6; n++) {
+        or_fields(p, s);
+        if (p->pressure) or_gradp(p, s->xb, s->detJac, s->gradP);
+        or_kk(p, s->b, n, s->dt, s->xb, s->B, s->J, s->gradP, s->kk, s->xb_tmp);
+        if (any_peri(p)) or_set_grid_peri(p, s->xb_tmp);
+        memcpy(s->xb, s->xb_tmp, P3 * sizeof(double));
+    }
+    double md = or_xnewstar(p, s->c, s->cs, s->xb, s->kk, s->xb_new, nan_seen);
+    if (any_peri(p)) or_set_grid_peri(p, s->xb_new);
+    return md;
+}
+
+/* src/main.cu:259-383 */
+int or_step(const or_params *p, or_state *s, double *dt_used, double *maxDelta, int *nan_seen)
+{
+    long P3 = 3 * npad(p);
+    double dt = s->dt;
+    double md = or_attempt(p, s, nan_seen);
+    if (dt_used) *dt_used = dt;
+    if (maxDelta) *maxDelta = md;
+    int acc = md < p->maxError;
+    if (acc) {
+        s->t += dt;
+        memcpy(s->xb, s->xb_new, P3 * sizeof(double));
+        or_fields(p, s);
+    }
+    if (md > 0) s->dt = dt * pow(0.9 * p->maxError / md, 0.2);
+    return acc;
+}

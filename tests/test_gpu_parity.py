@@ -1,0 +1,215 @@
+"""GPU parity against the CPU oracle, through the C ABI.
+
+FP64: every kernel of the reference's call structure, the fused stage
+kernel and whole steps must equal the oracle bit for bit (the library
+follows the reference's operation order and is built without FP
+contraction).  The one exception is dt after an attempt: the device
+controller's pow and the host's glibc pow may differ in the last bit, so dt
+is compared to 2 ulp and the oracle is then given the device's dt.
+"""
+import numpy as np
+import pytest
+
+from conftest import evolved_state, face_mask, inner_mask, small_braid
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g(require_gpu):
+    import glemur_amd
+    return glemur_amd
+
+
+def oracle_sim(p, B0, xb):
+    import oracle as o
+    return o.Sim(o.from_product(p), B0, xb)
+
+
+def assert_bitwise(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    bad = np.flatnonzero(~((a == b) | (np.isnan(a) & np.isnan(b))))
+    if bad.size:
+        i = bad[0]
+        rel = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+        raise AssertionError(f"{what}: {bad.size} of {a.size} differ; first at {i}: {a[i]!r} vs {b[i]!r}; "
+                             f"max rel {rel:.3e}")
+
+
+CASES = {
+    "plain": {},
+    "eps_profile": {"epsilonProf": True},
+    "pressure": {"pressure": True, "beta": 0.05},
+    "no_zupdate": {"zUpdate": False},
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_per_kernel_path_bitwise(g, case):
+    """B_JacB0, setBbound, JStokes, gradP, kk (six stages) and xNewStar, one
+    call each, against the oracle on a distorted grid."""
+    p = small_braid(**CASES[case])
+    B0, xb = evolved_state(p, 3)
+    sim = oracle_sim(p, B0, xb)
+    dt = 0.013
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, dt)
+        ctx.b_jacb0()
+        ctx.set_bbound()
+        sim.fields()
+        m_in, m_face = inner_mask(p), face_mask(p)
+        Bg, Bo = ctx.download(g.FIELD_B).reshape(-1, 3), sim.arr("B").reshape(-1, 3)
+        assert_bitwise(Bg[m_in], Bo[m_in], "B inner")
+        assert_bitwise(Bg[m_face], Bo[m_face], "B ghost faces")
+        dg, do = ctx.download(g.FIELD_DETJAC), sim.arr("detJac")
+        assert_bitwise(dg[m_in | m_face], do[m_in | m_face], "detJac")
+        ctx.current()
+        assert_bitwise(ctx.download(g.FIELD_J), sim.arr("J"), "J")
+        if p.pressure:
+            ctx.gradp()
+            sim.gradp()
+            assert_bitwise(ctx.download(g.FIELD_GRADP), sim.arr("gradP"), "gradP")
+        # full attempt through the per-kernel calls
+        sim.set_dt(dt)
+        md_o = sim.attempt()
+        for n in range(6):
+            if n > 0:
+                ctx.b_jacb0()
+                ctx.set_bbound()
+                ctx.current()
+                if p.pressure:
+                    ctx.gradp()
+            ctx.rk_kk(n, dt)
+        md_g = ctx.rk_xnewstar()
+        assert_bitwise(ctx.download(g.FIELD_KK), sim.arr("kk"), "kk")
+        assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), "x_6")
+        assert_bitwise(ctx.download(g.FIELD_XB_NEW), sim.arr("xb_new"), "xb_new")
+        assert md_g == md_o, (md_g, md_o)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fused_fields_bitwise(g, case):
+    """The fused stage kernel's B (inner and ghost faces), det and J for the
+    current positions (gm_fields) equal the oracle's."""
+    p = small_braid(**CASES[case])
+    B0, xb = evolved_state(p, 3)
+    sim = oracle_sim(p, B0, xb)
+    sim.fields()
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, 0.01)
+        ctx.fields()
+        m = inner_mask(p) | face_mask(p)
+        assert_bitwise(ctx.download(g.FIELD_B).reshape(-1, 3)[m], sim.arr("B").reshape(-1, 3)[m], "B")
+        assert_bitwise(ctx.download(g.FIELD_DETJAC)[m], sim.arr("detJac")[m], "detJac")
+        assert_bitwise(ctx.download(g.FIELD_J), sim.arr("J"), "J")
+        if p.pressure:
+            sim.gradp()
+            assert_bitwise(ctx.download(g.FIELD_GRADP), sim.arr("gradP"), "gradP")
+
+
+@pytest.mark.parametrize("tile", [None, (5, 3, 4), (7, 6, 23), (13, 2, 1)])
+@pytest.mark.parametrize("case", list(CASES))
+def test_fused_step_bitwise(g, case, tile):
+    """One fused attempted step (six stage launches + controller): stage
+    slopes, candidate positions, error max and accept decision bit for bit,
+    for several tile shapes (tile edges and z-chunk ends move)."""
+    p = small_braid(**CASES[case])
+    B0, xb = evolved_state(p, 2)
+    sim = oracle_sim(p, B0, xb)
+    dt = 0.011
+    sim.set_dt(dt)
+    acc_o, dtu_o, md_o, _ = sim.step()
+    with g.Context(p, g.PREC_F64) as ctx:
+        if tile:
+            ctx.set_tile(*tile)
+        ctx.upload(B0, xb, 0.0, dt)
+        s = ctx.step()
+        assert s.dt_used == dtu_o
+        assert s.maxDelta == md_o, (s.maxDelta, md_o)
+        assert bool(s.accepted) == acc_o
+        kk_g = ctx.download(g.FIELD_KK).reshape(6, -1)
+        kk_o = sim.arr("kk").reshape(6, -1)
+        assert_bitwise(kk_g[:5], kk_o[:5], "k_0..k_4")
+        xg = ctx.download(g.FIELD_XB)
+        # accepted: x_new; rejected: the stage-6 position, as the reference
+        assert_bitwise(xg, sim.arr("xb"), "positions after the attempt")
+        assert abs(s.dt - sim.dt) <= 2 * np.spacing(sim.dt), (s.dt, sim.dt)
+
+
+def test_fused_trajectory_bitwise(g):
+    """60 attempted steps of the small braid: every attempt's error max,
+    decision and positions bit for bit with the oracle run on the device's
+    dt sequence; dt itself within 2 ulp of the oracle's controller."""
+    p = small_braid(epsilonProf=True)
+    B0, xb = evolved_state(p, 0)
+    sim = oracle_sim(p, B0, xb)
+    n_rej = 0
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, p.dt0)
+        for it in range(60):
+            sim.set_dt(ctx.time()[1])
+            s = ctx.step()
+            acc, dtu, md, _ = sim.step()
+            assert s.dt_used == dtu
+            assert s.maxDelta == md, (it, s.maxDelta, md)
+            assert bool(s.accepted) == acc
+            assert abs(s.dt - sim.dt) <= 2 * np.spacing(sim.dt)
+            n_rej += not acc
+            if it % 20 == 19:
+                assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), f"positions at attempt {it}")
+        assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), "final positions")
+    assert n_rej >= 1, "trajectory should exercise the reject path"
+
+
+def test_reductions_match_oracle(g):
+    p = small_braid()
+    B0, xb = evolved_state(p, 3)
+    sim = oracle_sim(p, B0, xb)
+    sim.fields()
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb)
+        ctx.fields()
+        r = ctx.reductions()
+    assert r.B2 == pytest.approx(sim.b2(), rel=1e-13)
+    assert r.JMax == sim.jmax()
+    assert r.JxB_B2Max == sim.jxb_b2_max()
+    assert r.JxB_JBMax == sim.jxb_jb_max()
+
+
+@pytest.mark.parametrize("peri", ["xPeri", "yPeri", "zPeri"])
+def test_periodic_per_kernel_bitwise(g, peri):
+    """Periodic boundaries on the per-kernel path: B, detJac on the ghost
+    faces, the periodic copies and one whole attempt."""
+    p = small_braid(**{peri: True})
+    import oracle as o
+    q = o.from_product(p)
+    B0, xb = o.init_state(q)
+    sim = o.Sim(q, B0, xb)
+    for _ in range(2):
+        sim.step()
+    xb = sim.arr("xb").copy()
+    sim = o.Sim(q, B0, xb)
+    dt = 0.01
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, dt)
+        ctx.b_jacb0()
+        ctx.set_bbound()
+        ctx.set_peri_B()
+        ctx.current()
+        sim.fields()
+        m = inner_mask(p) | face_mask(p)
+        assert_bitwise(ctx.download(g.FIELD_B).reshape(-1, 3)[m], sim.arr("B").reshape(-1, 3)[m], "B")
+        assert_bitwise(ctx.download(g.FIELD_J), sim.arr("J"), "J")
+        sim.set_dt(dt)
+        md_o = sim.attempt()
+        for n in range(6):
+            if n > 0:
+                ctx.b_jacb0()
+                ctx.set_bbound()
+                ctx.set_peri_B()
+                ctx.current()
+            ctx.rk_kk(n, dt)
+        md_g = ctx.rk_xnewstar()
+        assert md_g == md_o
+        assert_bitwise(ctx.download(g.FIELD_XB_NEW), sim.arr("xb_new"), "xb_new")
