@@ -5,11 +5,6 @@
 
 namespace gmd {
 
-// Compiled tile capacity: a workgroup of kFusedThreads threads holds a rect
-// of (TX+4) x (TY+4) <= kFusedPts points per plane in LDS.
-constexpr int kFusedThreads = 512;
-constexpr int kFusedPts = 864;
-
 // Tableau as the fused kernels use it: b, c, cs of src/rungeKutta.cu:10-25
 // plus db[s][m] = b[s+1][m] - b[s][m] (the increments of the stage update,
 // src/rungeKutta.cu:219) and ce = c - cs (the error weights).
@@ -21,9 +16,15 @@ struct TableauF {
     double ce[6];
 };
 
+// Constant data of one line-tied ghost-face point: the two tangential
+// Jacobian columns (from the fixed ghost-layer positions, with the
+// reference's tile-edge rebuilding), the outer extrapolated ghost x2 and B0.
+struct GhostPt {
+    double tc0[3], tc1[3], x2[3], b0[3];
+};
+
 struct FusedArgs {
     Geo g;
-    TableauF tab;
     int stage;                        // 0..5, or -1 for fields only
     int TX, TY, CZ, tiles_x, tiles_y, nchunks, nblocks;
     int emit, diag, check_done;
@@ -39,6 +40,8 @@ struct FusedArgs {
     float *eB32, *eJ32, *eG32;
     double *edet;                     // emitted det (P)
     const double *epsx, *epsy, *epsz; // friction profile factors per axis
+    const GhostPt *ghost;             // [face][side][v][u], offsets in ghost_off
+    long long ghost_off[6];
     double *part_delta;               // [nblocks]
     double *part_diag;                // [4][nblocks]
     Ctl *ctl;
@@ -54,8 +57,13 @@ struct CtlArgs {
     int log_cap;
 };
 
-int fused_max_points();
-size_t fused_lds_bytes(int f32k);
+// Largest rect (TX+4)(TY+4) a workgroup holds: one point per thread.
+int fused_max_points(int f32k);
+void set_tableau(const TableauF &t);
+// Ghost-face constants from the (fixed) ghost positions and B0; X is the
+// current position buffer (F64 SoA, or hi/lo for F32K).
+void launch_ghost_setup(hipStream_t s, const FusedArgs &A, int f32k, GhostPt *out);
+long long ghost_points(const Geo &g, long long off[6]);
 void launch_fused(hipStream_t s, const FusedArgs &A, int f32k);
 void launch_controller(hipStream_t s, const CtlArgs &A);
 

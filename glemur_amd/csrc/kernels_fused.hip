@@ -3,38 +3,43 @@
 //
 // One launch computes one stage n of one attempted step over the whole
 // grid.  A workgroup owns an x-y tile of TX x TY inner points and marches
-// through a z-chunk of CZ planes.  Per plane it
-//   (b) forms the stage positions x_n = x0 + sum_{m<n} b_nm k_m on the tile
-//       plus a 2-point halo (loaded once from HBM, kept in an LDS ring of
-//       four planes),
-//   (d) evaluates B = jac.B0/det on the tile plus a 1-point face halo (LDS
-//       ring of three planes), with the line-tied boundary formula on
-//       ghost faces,
-//   (a) evaluates the Stokes current J, the force and the slope k_n on the
-//       tile points of the previous plane and writes k_n (or, in the last
-//       stage, the new positions and the error maximum).
+// through a z-chunk of CZ planes; each thread owns one column of the
+// (TX+4) x (TY+4) rect (tile plus a 2-point halo) for the whole march.  Per
+// plane p a thread
+//   (1) forms the stage position x_n(p+1) = x0 + sum_m b_nm k_m of its column
+//       from inputs loaded one iteration earlier and publishes it in an LDS
+//       ring of planes,
+//   (0) issues the HBM loads of plane p+2 (x0, earlier slopes, B0) into
+//       registers; they complete while the rest of the iteration computes,
+//   (2) evaluates B = jac.B0/det at (column, p) on the tile plus a 1-point
+//       face halo -- in-plane neighbours from LDS, z neighbours from its own
+//       register queue -- with the line-tied formula on ghost faces, and
+//       publishes it in an LDS ring of three planes,
+//   -- one workgroup barrier --
+//   (3) evaluates the Stokes current, the force and the slope k_n at
+//       (column, p-1) on tile points and writes k_n (the last stage writes
+//       the candidate positions and the error maximum instead).
 // So per point and stage HBM sees x0, B0 and the earlier slopes once and the
-// new slope once; B, J and x_n never go to HBM.
+// new slope once; B, J and x_n stay on chip.
 //
 // Two arithmetics: F64 follows the reference's operation order (incremental
-// stage positions of src/rungeKutta.cu:216-227 included) so it matches the
+// stage positions of src/rungeKutta.cu:216-227 included) and matches the
 // FP64 oracle bit for bit; F32K keeps positions as FP32 hi + lo words, forms
-// every position difference from hi and lo separately and centres the
-// Stokes circulation on the point's own field, with the determinant, error
-// norm, energy and dt in FP64.
+// position differences from hi and lo separately and centres the Stokes
+// circulation on the point's own field, with the determinant, error norm,
+// energy and dt in FP64.
 #include "gm_device.hpp"
 #include "gm_fused.hpp"
 
 namespace gmd {
 
+__constant__ TableauF c_tab;
+
 namespace {
 
-__device__ __forceinline__ int ring4(int p) { return (p + 8) & 3; }
-__device__ __forceinline__ int ring3(int p) { return (p + 9) % 3; }
-
 // Workgroup id -> logical tile so that the blocks one XCD receives
-// (hardware deals ids round-robin over the 8 XCDs) are a contiguous range of
-// tiles and share halos in that XCD's L2.
+// (hardware deals ids round-robin over the 8 XCDs) form a contiguous range
+// of tiles and share halos in that XCD's L2.
 __device__ __forceinline__ int xcd_remap(int bid, int nb)
 {
     int q = nb >> 3, r = nb & 7;
@@ -51,40 +56,74 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
     e = (a - (s - bb)) + (b - bb);
 }
 
-// A ghost-face point belongs to the tile that owns its nearest inner point.
-__device__ __forceinline__ bool owns(const Geo &g, int gi, int gj, int gk, int ix0, int ixe, int iy0, int iye, int Ks,
-                                     int Ke)
+__device__ __forceinline__ int nface(const Geo &g, int face) { return face == 0 ? g.ny : g.nx; }
+
+// B and det at a line-tied ghost-face point from its first inner neighbour
+// x_in along the normal and the precomputed constants (src/boundaries.cu:
+// 164-186): jac = [normal column from x_in and the outer ghost | tangential
+// columns], B_normal = B0_normal.
+__device__ __forceinline__ void ghost_b(const Geo &g, const GhostPt &G, int face, int side, const double xin[3],
+                                       double B[3], double &det)
 {
-    int ci = min(max(gi, 1), g.nx), cj = min(max(gj, 1), g.ny), ck = min(max(gk, 1), g.nz);
-    return ci - 1 >= ix0 && ci - 1 < ixe && cj - 1 >= iy0 && cj - 1 < iye && ck >= Ks && ck <= Ke;
+    const double h1 = face == 0 ? g.dx1 : face == 1 ? g.dy1 : g.dz1;
+    const int t0 = face == 0 ? 1 : 0, t1 = face == 2 ? 1 : 2;
+    double jac[3][3], d[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        d[l] = side ? G.x2[l] - xin[l] : xin[l] - G.x2[l];
+        jac[l][t0] = G.tc0[l];
+        jac[l][t1] = G.tc1[l];
+    }
+    jac_col(jac, face, d, h1);
+    double detJac1 = cauchy_b(jac, G.b0, face, B);
+    det = 1 / detJac1;
 }
 
-// ghost classification of a padded point: number of ghost coordinates and
-// the axis/side of the single one
-__device__ __forceinline__ int ghost_axis(const Geo &g, int gi, int gj, int gk, int &side)
+// ----------------------------------------------------------- ghost setup
+// Tangential Jacobian columns of a ghost-face point: in-plane neighbours are
+// read from the ghost layer, or rebuilt as x +- h (plus addDist on z faces)
+// at the edges of the reference's 16x16 launch tiles and at the last row
+// (src/boundaries.cu:70-110, 210-238, 338-366).
+template <bool F32>
+__global__ void k_ghost_setup(FusedArgs A, GhostPt *out)
 {
-    int cnt = 0, ax = -1;
-    side = 0;
-    if (gi == 0 || gi == g.nx + 1) { ++cnt; ax = 0; side = gi != 0; }
-    if (gj == 0 || gj == g.ny + 1) { ++cnt; ax = 1; side = gj != 0; }
-    if (gk == 0 || gk == g.nz + 1) { ++cnt; ax = 2; side = gk != 0; }
-    return cnt == 1 ? ax : (cnt == 0 ? -1 : -2);
-}
-
-// Line-tied ghost-face B (src/boundaries.cu:50-110,164-186 and the y/x face
-// twins), from double positions: xg ghost, xin first inner point along the
-// normal, in-plane stored neighbours um/up (first tangential axis) and vm/vp
-// (second).  (u, v) are the inner in-plane indices; tile-edge neighbours are
-// rebuilt as x +- h as the reference's 16x16 launch tiles do.
-__device__ void ghost_b(const Geo &g, int face, int side, int u, int v, const double xg[3], const double xin[3],
-                        const double um_s[3], const double up_s[3], const double vm_s[3], const double vp_s[3],
-                        const double b0[3], double B[3], double &det)
-{
+    const Geo &g = A.g;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long cnt[6];
+    long long total = 0;
+    for (int fs = 0; fs < 6; ++fs) {
+        int f = fs >> 1;
+        cnt[fs] = f == 0 ? (long long)g.ny * g.nz : f == 1 ? (long long)g.nx * g.nz : (long long)g.nx * g.ny;
+        total += cnt[fs];
+    }
+    if (t >= total) return;
+    int fs = 0;
+    long long r = t;
+    while (r >= cnt[fs]) r -= cnt[fs++];
+    const int face = fs >> 1, side = fs & 1;
+    const int nu = nface(g, face);
+    const int u = (int)(r % nu), v = (int)(r / nu);
     const int n = face, t0 = face == 0 ? 1 : 0, t1 = face == 2 ? 1 : 2;
     const int nn[3] = {g.nx, g.ny, g.nz};
+    const long long st[3] = {1, g.sx, g.sxy};
     const double hh[3] = {g.dx, g.dy, g.dz}, h1[3] = {g.dx1, g.dy1, g.dz1};
-    double x2[3] = {xg[0], xg[1], xg[2]};
-    double zg = 0;
+    int c[3];
+    c[n] = side ? nn[n] + 1 : 0;
+    c[t0] = u + 1;
+    c[t1] = v + 1;
+    const long long a = c[0] * st[0] + c[1] * st[1] + c[2] * st[2];
+    const int cur = A.ctl->cur;
+    auto pos = [&](long long q, double x[3]) {
+        for (int l = 0; l < 3; ++l) {
+            if (F32)
+                x[l] = (double)A.Xh[cur][l * g.P + q] + (double)A.Xl[cur][l * g.P + q];
+            else
+                x[l] = A.X64[cur][l * g.P + q];
+        }
+    };
+    double xg[3];
+    pos(a, xg);
+    double x2[3] = {xg[0], xg[1], xg[2]}, zg = 0;
     if (face == 2) {
         double zsec = 2 * g.dz * (2 * side - 1) + g.Oz + g.Lz * side;
         for (int l = 0; l < 2; ++l)
@@ -94,633 +133,612 @@ __device__ void ghost_b(const Geo &g, int face, int side, int u, int v, const do
     } else {
         x2[n] = xg[n] + (2 * side - 1) * hh[n];
     }
-    double L0[3], L2[3];
-    for (int l = 0; l < 3; ++l) {
-        L0[l] = side ? xin[l] : x2[l];
-        L2[l] = side ? x2[l] : xin[l];
-    }
-    auto nb = [&](double out[3], const double st[3], int axis, int sgn, bool edge, int du, int dv) {
+    auto nb = [&](double o[3], int axis, int sgn, bool edge, long long off, int du, int dv) {
         if (!edge) {
-            out[0] = st[0]; out[1] = st[1]; out[2] = st[2];
+            pos(a + off, o);
             return;
         }
-        out[0] = xg[0]; out[1] = xg[1]; out[2] = xg[2];
+        o[0] = xg[0]; o[1] = xg[1]; o[2] = xg[2];
         if (face == 2) {
-            out[0] = (axis == 0 ? xg[0] + sgn * g.dx : xg[0]) - add_dist(g.dist, u, v, zg, 0) +
-                     add_dist(g.dist, u + du, v + dv, zg, 0);
-            out[1] = (axis == 1 ? xg[1] + sgn * g.dy : xg[1]) - add_dist(g.dist, u, v, zg, 1) +
-                     add_dist(g.dist, u + du, v + dv, zg, 1);
+            o[0] = (axis == 0 ? xg[0] + sgn * g.dx : xg[0]) - add_dist(g.dist, u, v, zg, 0) +
+                   add_dist(g.dist, u + du, v + dv, zg, 0);
+            o[1] = (axis == 1 ? xg[1] + sgn * g.dy : xg[1]) - add_dist(g.dist, u, v, zg, 1) +
+                   add_dist(g.dist, u + du, v + dv, zg, 1);
         } else {
-            out[axis] = xg[axis] + sgn * hh[axis];
+            o[axis] = xg[axis] + sgn * hh[axis];
         }
     };
     double um[3], up[3], vm[3], vp[3];
-    nb(um, um_s, t0, -1, u % 16 == 0, -1, 0);
-    nb(up, up_s, t0, +1, (u % 16 == 15) || (u == nn[t0] - 1), +1, 0);
-    nb(vm, vm_s, t1, -1, v % 16 == 0, 0, -1);
-    nb(vp, vp_s, t1, +1, (v % 16 == 15) || (v == nn[t1] - 1), 0, +1);
+    nb(um, t0, -1, u % 16 == 0, -st[t0], -1, 0);
+    nb(up, t0, +1, (u % 16 == 15) || (u == nn[t0] - 1), st[t0], +1, 0);
+    nb(vm, t1, -1, v % 16 == 0, -st[t1], 0, -1);
+    nb(vp, t1, +1, (v % 16 == 15) || (v == nn[t1] - 1), st[t1], 0, +1);
+    GhostPt G;
     double jac[3][3], d[3];
-    for (int l = 0; l < 3; ++l) d[l] = L2[l] - L0[l];
-    jac_col(jac, n, d, h1[n]);
     for (int l = 0; l < 3; ++l) d[l] = up[l] - um[l];
     jac_col(jac, t0, d, h1[t0]);
     for (int l = 0; l < 3; ++l) d[l] = vp[l] - vm[l];
     jac_col(jac, t1, d, h1[t1]);
-    double detJac1 = cauchy_b(jac, b0, n, B);
-    det = 1 / detJac1;
+    for (int l = 0; l < 3; ++l) {
+        G.tc0[l] = jac[l][t0];
+        G.tc1[l] = jac[l][t1];
+        G.x2[l] = x2[l];
+        G.b0[l] = F32 ? (double)A.B0_32[l * g.P + a] : A.B0_64[l * g.P + a];
+    }
+    out[A.ghost_off[fs] + u + (long long)nu * v] = G;
 }
 
+// Per-precision position type and operations.
+template <bool F32>
+struct PrecT;
+template <>
+struct PrecT<false> {
+    using R = double;
+    struct Pos {
+        double x[3];
+    };
+    __device__ static double diff(const Pos &a, const Pos &b, int l) { return a.x[l] - b.x[l]; }
+    __device__ static double dbl(const Pos &a, int l) { return a.x[l]; }
+};
+template <>
+struct PrecT<true> {
+    using R = float;
+    struct Pos {
+        float h[3], s[3];
+    };
+    // compensated difference: hi parts nearly cancel exactly, the small
+    // parts carry the low-order bits
+    __device__ static float diff(const Pos &a, const Pos &b, int l) { return (a.h[l] - b.h[l]) + (a.s[l] - b.s[l]); }
+    __device__ static double dbl(const Pos &a, int l) { return (double)a.h[l] + (double)a.s[l]; }
+};
+
+
+// FP32 Stokes current from the six face neighbours, positions and fields
+// taken relative to the point itself (r_v = x_v - x_c, b_v = B_v - B_c).
+// For a closed quadrilateral v0..v3 the reference's circulation
+// sum_e (b_e+1 + b_e)/2 . (x_e+1 - x_e) equals 1/2 sum_e (b_e . r_e+1 - b_e+1 . r_e)
+// and its mean area vector 1/4 sum_e d_e x d_e+1 equals 1/2 (r2 - r0) x (r3 - r1);
+// both identities are exact in real arithmetic and take half the operations.
+// Order of the vertices as src/current.cu:291-382: xy W S E N, yz y- z- y+ z+,
+// xz z- x- z+ x+ (indices into xm, ym, xp, yp, zm, zp = 0..5).
+__device__ __forceinline__ void stokes_f32(const float r[6][3], const float b[6][3], float J[3])
+{
+#pragma clang fp contract(fast)
+    const int Q[3][4] = {{0, 1, 2, 3}, {1, 4, 3, 5}, {4, 0, 5, 2}};
+    float NA[3][3], II[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float I = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float *be = b[Q[q][e]], *bn = b[Q[q][(e + 1) & 3]];
+            const float *re = r[Q[q][e]], *rn = r[Q[q][(e + 1) & 3]];
+            I += (be[0] * rn[0] + be[1] * rn[1] + be[2] * rn[2]) - (bn[0] * re[0] + bn[1] * re[1] + bn[2] * re[2]);
+        }
+        II[q] = 0.5f * I;
+        const float *r0 = r[Q[q][0]], *r1 = r[Q[q][1]], *r2 = r[Q[q][2]], *r3 = r[Q[q][3]];
+        float u[3] = {r2[0] - r0[0], r2[1] - r0[1], r2[2] - r0[2]};
+        float v[3] = {r3[0] - r1[0], r3[1] - r1[1], r3[2] - r1[2]};
+        NA[q][0] = 0.5f * (u[1] * v[2] - u[2] * v[1]);
+        NA[q][1] = 0.5f * (u[2] * v[0] - u[0] * v[2]);
+        NA[q][2] = 0.5f * (u[0] * v[1] - u[1] * v[0]);
+    }
+    // J = NA^-1 II by the adjugate
+    float A00 = NA[1][1] * NA[2][2] - NA[1][2] * NA[2][1];
+    float A01 = NA[0][2] * NA[2][1] - NA[0][1] * NA[2][2];
+    float A02 = NA[0][1] * NA[1][2] - NA[0][2] * NA[1][1];
+    float A10 = NA[1][2] * NA[2][0] - NA[1][0] * NA[2][2];
+    float A11 = NA[0][0] * NA[2][2] - NA[0][2] * NA[2][0];
+    float A12 = NA[0][2] * NA[1][0] - NA[0][0] * NA[1][2];
+    float A20 = NA[1][0] * NA[2][1] - NA[1][1] * NA[2][0];
+    float A21 = NA[0][1] * NA[2][0] - NA[0][0] * NA[2][1];
+    float A22 = NA[0][0] * NA[1][1] - NA[0][1] * NA[1][0];
+    float det = NA[0][0] * A00 + NA[0][1] * A10 + NA[0][2] * A20;
+    float d1 = 1.0f / det;
+    J[0] = (A00 * II[0] + A01 * II[1] + A02 * II[2]) * d1;
+    J[1] = (A10 * II[0] + A11 * II[1] + A12 * II[2]) * d1;
+    J[2] = (A20 * II[0] + A21 * II[1] + A22 * II[2]) * d1;
+}
 }  // namespace
 
-// ============================================================== F64 kernel
-template <int NT, int NPTS>
-__global__ void __launch_bounds__(NT) k_fused_f64(FusedArgs A)
+// =============================================================== kernel
+// F32: FP32 stages with compensated positions (else FP64).  NT: threads =
+// largest rect.  STAGE: 0..5, or 6 for the fields of the current positions
+// only (no slope written).  PRESS: pressure force on.
+template <bool F32, int NT, int STAGE, bool PRESS>
+__global__ void __launch_bounds__(NT) k_stage(FusedArgs A)
 {
+    constexpr bool FINAL = STAGE == 5;
+    using PT = PrecT<F32>;
+    using R = typename PT::R;
+    using Pos = typename PT::Pos;
+    constexpr int NXL = 5;  // planes of positions kept in LDS (ring)
+
     extern __shared__ __align__(16) unsigned char smem[];
-    double *XL = reinterpret_cast<double *>(smem);            // [4][3][NPTS]
-    double *BL = XL + 4 * 3 * NPTS;                            // [3][3][NPTS]
+    // LDS rings: positions [NXL][3][NT] (hi and small part for F32), B [3][3][NT]
+    R *XLa = reinterpret_cast<R *>(smem);
+    R *XLb = XLa + (F32 ? NXL * 3 * NT : 0);
+    R *BL = XLa + (F32 ? 2 : 1) * NXL * 3 * NT;
     __shared__ double red[5][NT / 64];
 
     const Geo &g = A.g;
     Ctl *ctl = A.ctl;
     if (A.check_done && ctl->done) return;
     const int cur = ctl->cur;
-    const double dt = ctl->dt;
-    const int stage = A.stage;              // -1: fields only
-    const int n = stage < 0 ? 0 : stage;
-    const double *X = A.X64[cur];
-    const double *B0 = A.B0_64;
-    const double *K = A.K64;
+    const R dt = (R)ctl->dt;
+    constexpr int stage = STAGE == 6 ? -1 : STAGE;
+    constexpr int n = STAGE == 6 ? 0 : STAGE;
     const long long P = g.P, P3 = 3 * P;
 
-    int L = xcd_remap(blockIdx.x, A.nblocks);
-    int tx = L % A.tiles_x, ty = (L / A.tiles_x) % A.tiles_y, ch = L / (A.tiles_x * A.tiles_y);
+    const int L = xcd_remap(blockIdx.x, A.nblocks);
+    const int tx = L % A.tiles_x, ty = (L / A.tiles_x) % A.tiles_y, ch = L / (A.tiles_x * A.tiles_y);
     const int ix0 = tx * A.TX, iy0 = ty * A.TY;
     const int Ks = ch * A.CZ + 1, Ke = min(ch * A.CZ + A.CZ, g.nz);
-    const int RW = A.TX + 4, RH = A.TY + 4, npts = RW * RH;
-    const int ixe = min(ix0 + A.TX, g.nx), iye = min(iy0 + A.TY, g.ny);  // exclusive inner ends
+    const int RW = A.TX + 4, RH = A.TY + 4;
+    const int ixe = min(ix0 + A.TX, g.nx), iye = min(iy0 + A.TY, g.ny);
+
+    // ---- this thread's column
+    const int t = threadIdx.x;
+    const int c = t % RW, r = t / RW;
+    const int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
+    const bool col = t < RW * RH && gi >= 0 && gi <= g.nx + 1 && gj >= 0 && gj <= g.ny + 1;
+    const bool gx = gi == 0 || gi == g.nx + 1, gy = gj == 0 || gj == g.ny + 1;
+    const bool inB = col && c >= 1 && c <= RW - 2 && r >= 1 && r <= RH - 2 &&
+                     !((c == 1 || c == RW - 2) && (r == 1 || r == RH - 2)) && !(gx && gy);
+    const bool inJ = col && c >= 2 && c <= RW - 3 && r >= 2 && r <= RH - 3 && gi >= 1 && gi <= ixe && gj >= 1 &&
+                     gj <= iye;
+    const long long abase = (long long)gi + g.sx * (long long)gj;
+    const int ii = gi - 1, jj = gj - 1;
+    // ghost face of an x/y ghost column
+    const int cface = gx ? 0 : 1, cside = gx ? (gi != 0) : (gj != 0);
+    const int cu = gx ? jj : ii;
+    const double eps_xy = (g.epsilonProf && inJ) ? A.epsx[ii] * A.epsy[jj] : 1.0;
+    const R *Xh0 = F32 ? (const R *)A.Xh[cur] : (const R *)A.X64[cur];
+    const R *Xl0 = F32 ? (const R *)A.Xl[cur] : nullptr;
+    const R *B0 = F32 ? (const R *)A.B0_32 : (const R *)A.B0_64;
+    const R *K = F32 ? (const R *)A.K32 : (const R *)A.K64;
+
+    // ---- register state of the column
+    Pos xq[4];       // x_n at planes p-2 .. p+1
+    R bq[3][3];      // B at planes p-2 .. p
+    double dq[2];    // det at planes p-1, p
+    R b0q[2][3];     // B0 at planes p, p+1
+    // prefetched inputs of one plane
+    struct In {
+        R x[3], lo[3], k[5][3], b0[3];
+    } pf;
+    // F32 final stage: per plane lo word, sum c_m k_m and sum (c_m - cs_m) k_m over m < 5
+    float fl[3][3], fc[3][3];
+    double fe[3][3];
+    (void)fl; (void)fc; (void)fe;
 
     double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0;
     int nan_seen = 0;
 
-    // stage positions of one rect point on plane q into LDS slot ring4(q)
-    auto xn_plane = [&](int q) {
-        int s = ring4(q);
-        for (int pt = threadIdx.x; pt < npts; pt += NT) {
-            int c = pt % RW, r = pt / RW;
-            int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-            if (gi < 0 || gi > g.nx + 1 || gj < 0 || gj > g.ny + 1) continue;
-            long long a = gidx(g, gi, gj, q);
-            bool ghost = gi == 0 || gi == g.nx + 1 || gj == 0 || gj == g.ny + 1 || q == 0 || q == g.nz + 1;
-            for (int l = 0; l < 3; ++l) {
-                double x = X[l * P + a];
-                if (!ghost) {
-                    // x_n by the reference's incremental stage updates
-                    double kv[5];
-                    for (int m = 0; m < n; ++m) kv[m] = K[m * P3 + l * P + a];
-                    for (int s2 = 0; s2 < n; ++s2)
-                        for (int m = 0; m <= s2; ++m) x += A.tab.db[s2 * 6 + m] * kv[m];
+    auto plane_ok = [&](int q) { return q >= max(Ks - 2, 0) && q <= min(Ke + 2, g.nz + 1); };
+    auto ghost_pt = [&](int q) { return gx || gy || q == 0 || q == g.nz + 1; };
+    auto XLslot = [&](int q) { return ((q % NXL) + NXL) % NXL; };
+    auto ld_x = [&](int q, int pt, Pos &o) {
+        int s = XLslot(q);
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            if constexpr (F32) {
+                o.h[l] = XLa[(s * 3 + l) * NT + pt];
+                o.s[l] = XLb[(s * 3 + l) * NT + pt];
+            } else {
+                o.x[l] = XLa[(s * 3 + l) * NT + pt];
+            }
+        }
+    };
+    auto st_x = [&](int q, const Pos &v) {
+        int s = XLslot(q);
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            if constexpr (F32) {
+                XLa[(s * 3 + l) * NT + t] = v.h[l];
+                XLb[(s * 3 + l) * NT + t] = v.s[l];
+            } else {
+                XLa[(s * 3 + l) * NT + t] = v.x[l];
+            }
+        }
+    };
+    auto ld_b = [&](int q, int pt, R o[3]) {
+        int s = ((q % 3) + 3) % 3;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) o[l] = BL[(s * 3 + l) * NT + pt];
+    };
+
+    // (0) loads of plane q into `in`
+    auto load_plane = [&](int q, In &in) {
+        long long a = abase + g.sxy * (long long)q;
+        bool gh = ghost_pt(q);
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            in.x[l] = Xh0[l * P + a];
+            if constexpr (F32) in.lo[l] = Xl0[l * P + a];
+            in.b0[l] = B0[l * P + a];
+        }
+        if (!gh) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                if (m < n)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) in.k[m][l] = K[m * P3 + l * P + a];
+        }
+    };
+    // (1) stage position from the loaded inputs
+    auto form_x = [&](int q, const In &in, Pos &o, int fslot) {
+        bool gh = ghost_pt(q);
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            if constexpr (F32) {
+                float s = in.lo[l];
+                if (!gh)
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        if (m < n) s += (float)c_tab.b[n * 6 + m] * in.k[m][l];
+                o.h[l] = in.x[l];
+                o.s[l] = s;
+                if constexpr (FINAL) {
+                    float dc = 0;
+                    double er = 0;
+                    if (!gh)
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) {
+                            dc += (float)c_tab.c[m] * in.k[m][l];
+                            er += c_tab.ce[m] * (double)in.k[m][l];
+                        }
+                    fl[fslot][l] = in.lo[l];
+                    fc[fslot][l] = dc;
+                    fe[fslot][l] = er;
                 }
-                XL[(s * 3 + l) * NPTS + pt] = x;
+            } else {
+                // x_n by the reference's incremental stage updates
+                double x = in.x[l];
+                if (!gh)
+#pragma unroll
+                    for (int s2 = 0; s2 < 5; ++s2)
+                        if (s2 < n)
+#pragma unroll
+                            for (int m = 0; m <= s2; ++m) x += c_tab.db[s2 * 6 + m] * in.k[m][l];
+                o.x[l] = x;
             }
         }
     };
 
-    auto XLv = [&](int q, int l, int pt) { return XL[(ring4(q) * 3 + l) * NPTS + pt]; };
-    auto BLv = [&](int q, int l, int pt) { return BL[(ring3(q) * 3 + l) * NPTS + pt]; };
-
-    // prologue: planes Ks-2, Ks-1
-    for (int q = Ks - 2; q <= Ks - 1; ++q)
-        if (q >= 0) xn_plane(q);
-    __syncthreads();
-
-    for (int p = Ks - 1; p <= Ke + 1; ++p) {
-        // (b) positions two planes ahead of the current output plane
-        if (p + 1 <= g.nz + 1 && p + 1 <= Ke + 2) xn_plane(p + 1);
-        __syncthreads();
-
-        // (d) B on plane p over tile + face halo
-        if (p >= 0 && p <= g.nz + 1) {
-            int sb = ring3(p);
-            for (int pt = threadIdx.x; pt < npts; pt += NT) {
-                int c = pt % RW, r = pt / RW;
-                if (c < 1 || c > RW - 2 || r < 1 || r > RH - 2) continue;
-                if ((c == 1 || c == RW - 2) && (r == 1 || r == RH - 2)) continue;
-                int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-                if (gi < 0 || gi > g.nx + 1 || gj < 0 || gj > g.ny + 1) continue;
-                int side;
-                int ax = ghost_axis(g, gi, gj, p, side);
-                if (ax == -2) continue;
-                long long a = gidx(g, gi, gj, p);
-                double b0[3] = {B0[a], B0[P + a], B0[2 * P + a]};
-                double Bv[3];
-                if (ax == -1) {
-                    double jac[3][3], d[3];
-                    for (int l = 0; l < 3; ++l) d[l] = XLv(p, l, pt + 1) - XLv(p, l, pt - 1);
-                    jac_col(jac, 0, d, g.dx1);
-                    for (int l = 0; l < 3; ++l) d[l] = XLv(p, l, pt + RW) - XLv(p, l, pt - RW);
-                    jac_col(jac, 1, d, g.dy1);
-                    for (int l = 0; l < 3; ++l) d[l] = XLv(p + 1, l, pt) - XLv(p - 1, l, pt);
-                    jac_col(jac, 2, d, g.dz1);
-                    cauchy_b(jac, b0, -1, Bv);
+    // ---- march
+    const int p_first = Ks - 4, p_last = Ke + 1;
+    for (int p = p_first; p <= p_last; ++p) {
+        // (1) x_n(p+1) from the inputs loaded one iteration ago
+        if (col && plane_ok(p + 1)) {
+            form_x(p + 1, pf, xq[3], 2);
+            st_x(p + 1, xq[3]);
+#pragma unroll
+            for (int l = 0; l < 3; ++l) b0q[1][l] = pf.b0[l];
+        }
+        // (0) loads of plane p+2, in flight while B and J are computed
+        if (col && plane_ok(p + 2)) load_plane(p + 2, pf);
+        // (2) B(p)
+        if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1) {
+            const bool zg = p == 0 || p == g.nz + 1;
+            R Bv[3];
+            double detv = 0;
+            if (!gx && !gy && !zg) {
+                Pos xm, xp, ym, yp;
+                ld_x(p, t - 1, xm);
+                ld_x(p, t + 1, xp);
+                ld_x(p, t - RW, ym);
+                ld_x(p, t + RW, yp);
+                R jac[3][3], d[3];
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[l] = PT::diff(xp, xm, l);
+                jac_col(jac, 0, d, (R)g.dx1);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[l] = PT::diff(yp, ym, l);
+                jac_col(jac, 1, d, (R)g.dy1);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[l] = PT::diff(xq[3], xq[1], l);
+                jac_col(jac, 2, d, (R)g.dz1);
+                if constexpr (F32) {
+                    double jd[3][3];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) jd[a][b] = jac[a][b];
+                    detv = det3(jd);
+                    float rd = 1.0f / (float)detv;
+#pragma unroll
+                    for (int l = 0; l < 3; ++l)
+                        Bv[l] = (jac[l][0] * b0q[0][0] + jac[l][1] * b0q[0][1] + jac[l][2] * b0q[0][2]) * rd;
                 } else {
-                    double xg[3], xin[3], um[3], up[3], vm[3], vp[3], det;
-                    int pin, pum, pup, pvm, pvp, qin, qum, qup, qvm, qvp;
-                    // neighbour rect offsets and planes by face
-                    if (ax == 0) {          // x face: tangents y (rows), z (planes)
-                        pin = side ? pt - 1 : pt + 1; qin = p;
-                        pum = pt - RW; qum = p; pup = pt + RW; qup = p;
-                        pvm = pt; qvm = p - 1; pvp = pt; qvp = p + 1;
-                    } else if (ax == 1) {   // y face: tangents x (cols), z (planes)
-                        pin = side ? pt - RW : pt + RW; qin = p;
-                        pum = pt - 1; qum = p; pup = pt + 1; qup = p;
-                        pvm = pt; qvm = p - 1; pvp = pt; qvp = p + 1;
-                    } else {                // z face: tangents x, y
-                        pin = pt; qin = side ? p - 1 : p + 1;
-                        pum = pt - 1; qum = p; pup = pt + 1; qup = p;
-                        pvm = pt - RW; qvm = p; pvp = pt + RW; qvp = p;
-                    }
-                    int u = ax == 0 ? gj - 1 : gi - 1, v = ax == 2 ? gj - 1 : p - 1;
-                    bool eum = u % 16 == 0, eup = (u % 16 == 15) || (u == (ax == 0 ? g.ny : g.nx) - 1);
-                    int nv = ax == 2 ? g.ny : g.nz;
-                    bool evm = v % 16 == 0, evp = (v % 16 == 15) || (v == nv - 1);
-                    for (int l = 0; l < 3; ++l) {
-                        xg[l] = XLv(p, l, pt);
-                        xin[l] = XLv(qin, l, pin);
-                        um[l] = eum ? 0 : XLv(qum, l, pum);
-                        up[l] = eup ? 0 : XLv(qup, l, pup);
-                        vm[l] = evm ? 0 : XLv(qvm, l, pvm);
-                        vp[l] = evp ? 0 : XLv(qvp, l, pvp);
-                    }
-                    ghost_b(g, ax, side, u, v, xg, xin, um, up, vm, vp, b0, Bv, det);
-                    if (A.emit && owns(g, gi, gj, p, ix0, ixe, iy0, iye, Ks, Ke)) {
-                        for (int l = 0; l < 3; ++l) A.eB64[l * P + a] = Bv[l];
-                        A.edet[a] = det;
+                    double detJac1 = cauchy_b(jac, b0q[0], -1, Bv);
+                    detv = 1. / detJac1;
+                }
+            } else {
+                // single ghost coordinate: line-tied face formula
+                int face, side, u, v;
+                double xin[3];
+                if (zg) {
+                    face = 2;
+                    side = p != 0;
+                    u = ii;
+                    v = jj;
+                    const Pos &pin = side ? xq[1] : xq[3];
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) xin[l] = PT::dbl(pin, l);
+                } else {
+                    face = cface;
+                    side = cside;
+                    u = cu;
+                    v = p - 1;
+                    int off = face == 0 ? (side ? -1 : 1) : (side ? -RW : RW);
+                    Pos pin;
+                    ld_x(p, t + off, pin);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) xin[l] = PT::dbl(pin, l);
+                }
+                const GhostPt &G = A.ghost[A.ghost_off[face * 2 + side] + u + (long long)nface(g, face) * v];
+                double Bd[3];
+                ghost_b(g, G, face, side, xin, Bd, detv);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) Bv[l] = (R)Bd[l];
+                if (A.emit) {
+                    int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
+                    if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
+                        long long a = abase + g.sxy * (long long)p;
+#pragma unroll
+                        for (int l = 0; l < 3; ++l) {
+                            if constexpr (F32) A.eB32[l * P + a] = Bv[l];
+                            else A.eB64[l * P + a] = Bv[l];
+                        }
+                        A.edet[a] = detv;
                     }
                 }
-                for (int l = 0; l < 3; ++l) BL[(sb * 3 + l) * NPTS + pt] = Bv[l];
             }
+            int sb = ((p % 3) + 3) % 3;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                bq[2][l] = Bv[l];
+                BL[(sb * 3 + l) * NT + t] = Bv[l];
+            }
+            dq[1] = detv;
         }
         __syncthreads();
 
-        // (a) J, force and slope on plane pj = p - 1
+        // (3) J, force and slope at (column, p-1)
         const int pj = p - 1;
-        if (pj >= Ks && pj <= Ke) {
-            for (int pt = threadIdx.x; pt < npts; pt += NT) {
-                int c = pt % RW, r = pt / RW;
-                int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-                if (c < 2 || c > RW - 3 || r < 2 || r > RH - 3) continue;
-                if (gi < 1 || gi > ixe || gj < 1 || gj > iye) continue;
-                long long a = gidx(g, gi, gj, pj);
-                const int nb4[3][4][2] = {
-                    // {rect offset kind, plane offset}: 0:-x 1:-y 2:+x 3:+y 4:-z 5:+z
-                    {{0, 0}, {1, 0}, {2, 0}, {3, 0}},
-                    {{1, 0}, {4, -1}, {3, 0}, {5, 1}},
-                    {{4, -1}, {0, 0}, {5, 1}, {2, 0}}};
-                const int off[6] = {-1, -RW, +1, +RW, 0, 0};
-                double NA[3][3], II[3];
-                for (int q = 0; q < 3; ++q) {
-                    double x[4][3], bv[4][3], d[4][3];
-                    for (int e = 0; e < 4; ++e) {
-                        int kind = nb4[q][e][0], dq = nb4[q][e][1];
-                        int pp = pt + off[kind];
-                        for (int l = 0; l < 3; ++l) {
-                            x[e][l] = XLv(pj + dq, l, pp);
-                            bv[e][l] = BLv(pj + dq, l, pp);
-                        }
-                    }
+        if (inJ && pj >= Ks && pj <= Ke) {
+            const long long a = abase + g.sxy * (long long)pj;
+            Pos xm, xp, ym, yp;
+            ld_x(pj, t - 1, xm);
+            ld_x(pj, t + 1, xp);
+            ld_x(pj, t - RW, ym);
+            ld_x(pj, t + RW, yp);
+            const Pos &zm = xq[0], &zp = xq[2];
+            R Bxm[3], Bxp[3], Bym[3], Byp[3];
+            ld_b(pj, t - 1, Bxm);
+            ld_b(pj, t + 1, Bxp);
+            ld_b(pj, t - RW, Bym);
+            ld_b(pj, t + RW, Byp);
+            const R *Bzm = bq[0], *Bzp = bq[2], *Bc = bq[1];
+            R Jv[3], kv[3];
+            if constexpr (F32) {
+                const Pos &xc = xq[1];
+                const Pos *V[6] = {&xm, &ym, &xp, &yp, &zm, &zp};
+                const float *W[6] = {Bxm, Bym, Bxp, Byp, Bzm, Bzp};
+                float rr[6][3], bb[6][3];
+#pragma unroll
+                for (int v = 0; v < 6; ++v)
+#pragma unroll
                     for (int l = 0; l < 3; ++l) {
-                        d[0][l] = x[1][l] - x[0][l];
-                        d[1][l] = x[2][l] - x[1][l];
-                        d[2][l] = x[3][l] - x[2][l];
-                        d[3][l] = x[0][l] - x[3][l];
+                        rr[v][l] = PT::diff(*V[v], xc, l);
+                        bb[v][l] = W[v][l] - Bc[l];
                     }
+                stokes_f32(rr, bb, Jv);
+            } else {
+                // diamonds in circulation order (src/current.cu:291-382):
+                // xy: W S E N; yz: y- z- y+ z+; xz: z- x- z+ x+
+                const Pos *V[3][4] = {{&xm, &ym, &xp, &yp}, {&ym, &zm, &yp, &zp}, {&zm, &xm, &zp, &xp}};
+                const R *W[3][4] = {{Bxm, Bym, Bxp, Byp}, {Bym, Bzm, Byp, Bzp}, {Bzm, Bxm, Bzp, Bxp}};
+                R NA[3][3], II[3];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    R d[4][3], bv[4][3];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int l = 0; l < 3; ++l) {
+                            d[e][l] = PT::diff(*V[q][(e + 1) & 3], *V[q][e], l);
+                            bv[e][l] = W[q][e][l];
+                        }
                     II[q] = diamond(d, bv, NA[q]);
                 }
-                double Jv[3], Bc[3], kv[3];
                 solve_na(NA, II, Jv);
-                for (int l = 0; l < 3; ++l) Bc[l] = BLv(pj, l, pt);
-                cross3(Jv, Bc, kv);
-                int ii = gi - 1, jj = gj - 1, kk = pj - 1;
-                double eps = g.epsilonProf ? A.epsx[ii] * A.epsy[jj] * A.epsz[kk] : 1.0;
-                double gv[3] = {0, 0, 0};
-                double detv = 0;
-                if (g.pressure || A.emit) {
-                    auto x = [&](int di, int dj, int dk, int l) { return XLv(pj + dk, l, pt + di + dj * RW); };
-                    double jac[3][3];
-                    for (int l = 0; l < 3; ++l) {
-                        jac[l][0] = (x(1, 0, 0, l) - x(-1, 0, 0, l)) * g.dx1 / 2;
-                        jac[l][1] = (x(0, 1, 0, l) - x(0, -1, 0, l)) * g.dy1 / 2;
-                        jac[l][2] = (x(0, 0, 1, l) - x(0, 0, -1, l)) * g.dz1 / 2;
+            }
+            cross3(Jv, Bc, kv);
+            const int kk = pj - 1;
+            R eps = (R)1;
+            if (g.epsilonProf) eps = (R)(eps_xy * A.epsz[kk]);
+            R gv[3] = {0, 0, 0};
+            const double detv = dq[0];
+            if constexpr (PRESS) {
+                // gradP from second differences (src/gradP.cu:93-154)
+                auto xat = [&](int di, int dj, int dk, Pos &o) {
+                    if (di == 0 && dj == 0) {
+                        o = dk < 0 ? xq[0] : dk > 0 ? xq[2] : xq[1];
+                    } else {
+                        ld_x(pj + dk, t + di + dj * RW, o);
                     }
-                    double detJac1 = 1. / det3(jac);
-                    detv = 1. / detJac1;
-                    if (g.pressure) {
-                        double d2[3][3][3];
-                        for (int l = 0; l < 3; ++l) {
-                            d2[l][0][0] = (x(1, 0, 0, l) - 2 * x(0, 0, 0, l) + x(-1, 0, 0, l)) * g.dx1 * g.dx1;
-                            d2[l][1][1] = (x(0, 1, 0, l) - 2 * x(0, 0, 0, l) + x(0, -1, 0, l)) * g.dy1 * g.dy1;
-                            d2[l][2][2] = (x(0, 0, 1, l) - 2 * x(0, 0, 0, l) + x(0, 0, -1, l)) * g.dz1 * g.dz1;
-                            d2[l][0][1] = (x(1, 1, 0, l) - x(1, -1, 0, l) - x(-1, 1, 0, l) + x(-1, -1, 0, l)) * g.dx1 * g.dy1 / 4;
-                            d2[l][0][2] = (x(1, 0, 1, l) - x(1, 0, -1, l) - x(-1, 0, 1, l) + x(-1, 0, -1, l)) * g.dx1 * g.dz1 / 4;
-                            d2[l][1][2] = (x(0, 1, 1, l) - x(0, -1, 1, l) - x(0, 1, -1, l) + x(0, -1, -1, l)) * g.dy1 * g.dz1 / 4;
-                            d2[l][1][0] = d2[l][0][1];
-                            d2[l][2][0] = d2[l][0][2];
-                            d2[l][2][1] = d2[l][1][2];
-                        }
-                        double det1 = 1 / detv;
-                        gradp_core(jac, d2, det1, g.beta, gv);
+                };
+                Pos pa, pb, pc, pd;
+                R jac[3][3], d2[3][3][3], d[3];
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[l] = PT::diff(xp, xm, l);
+                jac_col(jac, 0, d, (R)g.dx1);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[l] = PT::diff(yp, ym, l);
+                jac_col(jac, 1, d, (R)g.dy1);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[l] = PT::diff(zp, zm, l);
+                jac_col(jac, 2, d, (R)g.dz1);
+                const Pos &x0c = xq[1];
+                const R h1[3] = {(R)g.dx1, (R)g.dy1, (R)g.dz1};
+                const Pos *ax[3][2] = {{&xm, &xp}, {&ym, &yp}, {&zm, &zp}};
+#pragma unroll
+                for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) {
+                        if constexpr (F32)
+                            d2[l][a2][a2] = (PT::diff(*ax[a2][1], x0c, l) - PT::diff(x0c, *ax[a2][0], l)) * h1[a2] * h1[a2];
+                        else
+                            d2[l][a2][a2] = (ax[a2][1]->x[l] - 2 * x0c.x[l] + ax[a2][0]->x[l]) * h1[a2] * h1[a2];
+                    }
+                // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b)) for (a,b) = (0,1),(0,2),(1,2)
+                const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    int aa = pr[m][0], bb = pr[m][1];
+                    int o1[3] = {0, 0, 0}, o2[3] = {0, 0, 0};
+                    o1[aa] = 1;
+                    o2[bb] = 1;
+                    xat(o1[0] + o2[0], o1[1] + o2[1], o1[2] + o2[2], pa);
+                    xat(o1[0] - o2[0], o1[1] - o2[1], o1[2] - o2[2], pb);
+                    xat(-o1[0] + o2[0], -o1[1] + o2[1], -o1[2] + o2[2], pc);
+                    xat(-o1[0] - o2[0], -o1[1] - o2[1], -o1[2] - o2[2], pd);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) {
+                        R v;
+                        if constexpr (F32)
+                            v = (PT::diff(pa, pb, l) - PT::diff(pc, pd, l)) * h1[aa] * h1[bb] / 4;
+                        else if (m == 2)  // the reference's term order for d2/dy dz
+                            v = (pa.x[l] - pc.x[l] - pb.x[l] + pd.x[l]) * h1[aa] * h1[bb] / 4;
+                        else
+                            v = (pa.x[l] - pb.x[l] - pc.x[l] + pd.x[l]) * h1[aa] * h1[bb] / 4;
+                        d2[l][aa][bb] = v;
+                        d2[l][bb][aa] = v;
                     }
                 }
+                R det1 = (R)(1 / detv);
+                gradp_core(jac, d2, det1, (R)g.beta, gv);
+            }
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                kv[l] *= dt * eps;
+                if (PRESS) kv[l] += -dt * gv[l];
+            }
+            if (A.emit) {
+#pragma unroll
                 for (int l = 0; l < 3; ++l) {
-                    kv[l] *= dt * eps;
-                    if (g.pressure) kv[l] += -dt * gv[l];
-                }
-                if (A.emit) {
-                    for (int l = 0; l < 3; ++l) {
+                    if constexpr (F32) {
+                        A.eB32[l * P + a] = Bc[l];
+                        A.eJ32[l * P + a] = Jv[l];
+                        if (PRESS) A.eG32[l * P + a] = gv[l];
+                    } else {
                         A.eB64[l * P + a] = Bc[l];
                         A.eJ64[l * P + a] = Jv[l];
-                        if (g.pressure) A.eG64[l * P + a] = gv[l];
+                        if (PRESS) A.eG64[l * P + a] = gv[l];
                     }
-                    A.edet[a] = detv;
                 }
-                if (A.diag) {
-                    double jxb[3];
-                    cross3(Jv, Bc, jxb);
-                    double b2 = Bc[0] * Bc[0] + Bc[1] * Bc[1] + Bc[2] * Bc[2];
-                    double nc = sqrt(jxb[0] * jxb[0] + jxb[1] * jxb[1] + jxb[2] * jxb[2]);
-                    double nj = sqrt(Jv[0] * Jv[0] + Jv[1] * Jv[1] + Jv[2] * Jv[2]);
-                    double dv = detv;
-                    if (!(g.pressure || A.emit)) {
-                        // det at the point for the energy weight
-                        auto x = [&](int di, int dj, int dk, int l) { return XLv(pj + dk, l, pt + di + dj * RW); };
-                        double jac[3][3];
-                        for (int l = 0; l < 3; ++l) {
-                            jac[l][0] = (x(1, 0, 0, l) - x(-1, 0, 0, l)) * g.dx1 / 2;
-                            jac[l][1] = (x(0, 1, 0, l) - x(0, -1, 0, l)) * g.dy1 / 2;
-                            jac[l][2] = (x(0, 0, 1, l) - x(0, 0, -1, l)) * g.dz1 / 2;
-                        }
-                        dv = 1. / (1. / det3(jac));
-                    }
-                    double v = b2 * dv;
-                    int bound = (ii == 0) + (jj == 0) + (kk == 0) + (ii == g.nx - 1) + (jj == g.ny - 1) + (kk == g.nz - 1);
-                    v /= (1 << bound);
-                    e_sum += v;
-                    jmax = nb_max(jmax, nj);
-                    r1 = nb_max(r1, nc / b2);
-                    double den = nj * sqrt(b2);
-                    r2 = nb_max(r2, den > 0 ? nc / den : 0.0);
+                A.edet[a] = detv;
+            }
+            if (A.diag) {
+                double Bd[3] = {(double)Bc[0], (double)Bc[1], (double)Bc[2]};
+                double Jd[3] = {(double)Jv[0], (double)Jv[1], (double)Jv[2]}, jxb[3];
+                cross3(Jd, Bd, jxb);
+                double b2 = Bd[0] * Bd[0] + Bd[1] * Bd[1] + Bd[2] * Bd[2];
+                double nc = sqrt(jxb[0] * jxb[0] + jxb[1] * jxb[1] + jxb[2] * jxb[2]);
+                double nj = sqrt(Jd[0] * Jd[0] + Jd[1] * Jd[1] + Jd[2] * Jd[2]);
+                double v = b2 * detv;
+                int bound = (ii == 0) + (jj == 0) + (kk == 0) + (ii == g.nx - 1) + (jj == g.ny - 1) + (kk == g.nz - 1);
+                v /= (1 << bound);
+                e_sum += v;
+                jmax = nb_max(jmax, nj);
+                r1 = nb_max(r1, nc / b2);
+                double den = nj * sqrt(b2);
+                r2 = nb_max(r2, den > 0 ? nc / den : 0.0);
+            }
+            if constexpr (!FINAL) {
+                if constexpr (stage >= 0) {
+                    R *Ko = F32 ? (R *)A.K32 : (R *)A.K64;
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) Ko[stage * P3 + l * P + a] = kv[l];
                 }
-                if (stage >= 0 && stage < 5) {
-                    for (int l = 0; l < 3; ++l) A.K64[stage * P3 + l * P + a] = kv[l];
-                } else if (stage == 5) {
-                    // x_6 = x_5 + sum_m (b6m - b5m) k_m, then x5/x4 weights
-                    // (src/rungeKutta.cu:216-227, 296-309)
-                    // x_new to buffer cur+1; the stage-6 position x_6 (where a
-                    // rejected attempt leaves xb, src/main.cu:288,378-383) to cur+2
-                    double *Xd = A.X64[(cur + 1) % 3], *X6 = A.X64[(cur + 2) % 3];
-                    for (int l = 0; l < 3; ++l) {
-                        double k6[6];
-                        for (int m = 0; m < 5; ++m) k6[m] = K[m * P3 + l * P + a];
-                        k6[5] = kv[l];
-                        double x6 = XLv(pj, l, pt);
-                        for (int m = 0; m < 6; ++m) x6 += A.tab.db[5 * 6 + m] * k6[m];
-                        X6[l * P + a] = x6;
-                        double xn = x6, xs = x6;
-                        if (l < 2 || g.zUpdate)
-                            for (int m = 0; m < 6; ++m) {
-                                xn += A.tab.c[m] * k6[m];
-                                xs += A.tab.cs[m] * k6[m];
-                            }
-                        Xd[l * P + a] = xn;
-                        double dd = fabs(xn - xs);
+            } else if constexpr (F32) {
+                float *Xhd = A.Xh[cur ^ 1], *Xld = A.Xl[cur ^ 1];
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    float hi = xq[1].h[l], lo = fl[0][l];
+                    if (l < 2 || g.zUpdate) {
+                        float dC = fc[0][l] + (float)c_tab.c[5] * kv[l];
+                        double err = fe[0][l] + c_tab.ce[5] * (double)kv[l];
+                        // (hi, lo) + dC, keeping the rounding error of hi + dC
+                        float s, e;
+                        two_sum(hi, dC, s, e);
+                        float lo2 = lo + e;
+                        float hi2 = s + lo2;
+                        lo = lo2 - (hi2 - s);
+                        hi = hi2;
+                        double dd = fabs(err);
                         if (dd != dd) nan_seen = 1;
                         dmax = nb_max(dmax, dd);
                     }
+                    Xhd[l * P + a] = hi;
+                    Xld[l * P + a] = lo;
                 }
-            }
-        }
-        __syncthreads();
-    }
-
-    // block partials
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (stage == 5) {
-        dmax = wave_max(dmax);
-        if (lane == 0) red[0][w] = dmax;
-        if (nan_seen) atomicOr(&ctl->nan_seen, 1);
-    }
-    if (A.diag) {
-        e_sum = wave_sum(e_sum);
-        jmax = wave_max(jmax);
-        r1 = wave_max(r1);
-        r2 = wave_max(r2);
-        if (lane == 0) {
-            red[1][w] = e_sum;
-            red[2][w] = jmax;
-            red[3][w] = r1;
-            red[4][w] = r2;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (stage == 5) {
-            double m = 0;
-            for (int q = 0; q < NT / 64; ++q) m = nb_max(m, red[0][q]);
-            A.part_delta[blockIdx.x] = m;
-        }
-        if (A.diag) {
-            double s = 0, a1 = 0, a2 = 0, a3 = 0;
-            for (int q = 0; q < NT / 64; ++q) {
-                s += red[1][q];
-                a1 = nb_max(a1, red[2][q]);
-                a2 = nb_max(a2, red[3][q]);
-                a3 = nb_max(a3, red[4][q]);
-            }
-            A.part_diag[blockIdx.x] = s;
-            A.part_diag[A.nblocks + blockIdx.x] = a1;
-            A.part_diag[2 * A.nblocks + blockIdx.x] = a2;
-            A.part_diag[3 * A.nblocks + blockIdx.x] = a3;
-        }
-    }
-}
-
-// ============================================================= F32K kernel
-template <int NT, int NPTS>
-__global__ void __launch_bounds__(NT) k_fused_f32k(FusedArgs A)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    float *XH = reinterpret_cast<float *>(smem);     // [4][3][NPTS]
-    float *XS = XH + 4 * 3 * NPTS;                    // [4][3][NPTS]
-    float *BL = XS + 4 * 3 * NPTS;                    // [3][3][NPTS]
-    __shared__ double red[5][NT / 64];
-
-    const Geo &g = A.g;
-    Ctl *ctl = A.ctl;
-    if (A.check_done && ctl->done) return;
-    const int cur = ctl->cur;
-    const float dt = (float)ctl->dt;
-    const int stage = A.stage;
-    const int n = stage < 0 ? 0 : stage;
-    const float *XHg = A.Xh[cur], *XLg = A.Xl[cur];
-    const float *B0 = A.B0_32;
-    const float *K = A.K32;
-    const long long P = g.P, P3 = 3 * P;
-    const float h1x = (float)g.dx1, h1y = (float)g.dy1, h1z = (float)g.dz1;
-
-    int L = xcd_remap(blockIdx.x, A.nblocks);
-    int tx = L % A.tiles_x, ty = (L / A.tiles_x) % A.tiles_y, ch = L / (A.tiles_x * A.tiles_y);
-    const int ix0 = tx * A.TX, iy0 = ty * A.TY;
-    const int Ks = ch * A.CZ + 1, Ke = min(ch * A.CZ + A.CZ, g.nz);
-    const int RW = A.TX + 4, RH = A.TY + 4, npts = RW * RH;
-    const int ixe = min(ix0 + A.TX, g.nx), iye = min(iy0 + A.TY, g.ny);
-
-    float bn[5];
-    for (int m = 0; m < 5; ++m) bn[m] = (float)A.tab.b[n * 6 + m];
-
-    double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0;
-    int nan_seen = 0;
-
-    auto xn_plane = [&](int q) {
-        int s = ring4(q);
-        for (int pt = threadIdx.x; pt < npts; pt += NT) {
-            int c = pt % RW, r = pt / RW;
-            int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-            if (gi < 0 || gi > g.nx + 1 || gj < 0 || gj > g.ny + 1) continue;
-            long long a = gidx(g, gi, gj, q);
-            bool ghost = gi == 0 || gi == g.nx + 1 || gj == 0 || gj == g.ny + 1 || q == 0 || q == g.nz + 1;
-            for (int l = 0; l < 3; ++l) {
-                float sv = XLg[l * P + a];
-                if (!ghost)
-                    for (int m = 0; m < n; ++m) sv += bn[m] * K[m * P3 + l * P + a];
-                XH[(s * 3 + l) * NPTS + pt] = XHg[l * P + a];
-                XS[(s * 3 + l) * NPTS + pt] = sv;
-            }
-        }
-    };
-    auto xh = [&](int q, int l, int pt) { return XH[(ring4(q) * 3 + l) * NPTS + pt]; };
-    auto xs = [&](int q, int l, int pt) { return XS[(ring4(q) * 3 + l) * NPTS + pt]; };
-    // compensated difference x(qa,pa) - x(qb,pb)
-    auto dif = [&](int qa, int pa, int qb, int pb, int l) {
-        return (xh(qa, l, pa) - xh(qb, l, pb)) + (xs(qa, l, pa) - xs(qb, l, pb));
-    };
-    auto xd = [&](int q, int l, int pt) { return (double)xh(q, l, pt) + (double)xs(q, l, pt); };
-    auto BLv = [&](int q, int l, int pt) { return BL[(ring3(q) * 3 + l) * NPTS + pt]; };
-
-    for (int q = Ks - 2; q <= Ks - 1; ++q)
-        if (q >= 0) xn_plane(q);
-    __syncthreads();
-
-    for (int p = Ks - 1; p <= Ke + 1; ++p) {
-        if (p + 1 <= g.nz + 1 && p + 1 <= Ke + 2) xn_plane(p + 1);
-        __syncthreads();
-
-        if (p >= 0 && p <= g.nz + 1) {
-            int sb = ring3(p);
-            for (int pt = threadIdx.x; pt < npts; pt += NT) {
-                int c = pt % RW, r = pt / RW;
-                if (c < 1 || c > RW - 2 || r < 1 || r > RH - 2) continue;
-                if ((c == 1 || c == RW - 2) && (r == 1 || r == RH - 2)) continue;
-                int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-                if (gi < 0 || gi > g.nx + 1 || gj < 0 || gj > g.ny + 1) continue;
-                int side;
-                int ax = ghost_axis(g, gi, gj, p, side);
-                if (ax == -2) continue;
-                long long a = gidx(g, gi, gj, p);
-                float Bv[3];
-                if (ax == -1) {
-                    float jac[3][3], d[3];
-                    for (int l = 0; l < 3; ++l) d[l] = dif(p, pt + 1, p, pt - 1, l);
-                    jac_col(jac, 0, d, h1x);
-                    for (int l = 0; l < 3; ++l) d[l] = dif(p, pt + RW, p, pt - RW, l);
-                    jac_col(jac, 1, d, h1y);
-                    for (int l = 0; l < 3; ++l) d[l] = dif(p + 1, pt, p - 1, pt, l);
-                    jac_col(jac, 2, d, h1z);
-                    double jd[3][3];
-                    for (int u = 0; u < 3; ++u)
-                        for (int v = 0; v < 3; ++v) jd[u][v] = jac[u][v];
-                    double detJac1 = 1.0 / det3(jd);
-                    float b0[3] = {B0[a], B0[P + a], B0[2 * P + a]};
-                    for (int l = 0; l < 3; ++l)
-                        Bv[l] = (float)((double)(jac[l][0] * b0[0] + jac[l][1] * b0[1] + jac[l][2] * b0[2]) * detJac1);
-                } else {
-                    double xg[3], xin[3], um[3], up[3], vm[3], vp[3], b0[3], Bd[3], det;
-                    int pin, pum, pup, pvm, pvp, qin, qum, qup, qvm, qvp;
-                    if (ax == 0) {
-                        pin = side ? pt - 1 : pt + 1; qin = p;
-                        pum = pt - RW; qum = p; pup = pt + RW; qup = p;
-                        pvm = pt; qvm = p - 1; pvp = pt; qvp = p + 1;
-                    } else if (ax == 1) {
-                        pin = side ? pt - RW : pt + RW; qin = p;
-                        pum = pt - 1; qum = p; pup = pt + 1; qup = p;
-                        pvm = pt; qvm = p - 1; pvp = pt; qvp = p + 1;
-                    } else {
-                        pin = pt; qin = side ? p - 1 : p + 1;
-                        pum = pt - 1; qum = p; pup = pt + 1; qup = p;
-                        pvm = pt - RW; qvm = p; pvp = pt + RW; qvp = p;
-                    }
-                    int u = ax == 0 ? gj - 1 : gi - 1, v = ax == 2 ? gj - 1 : p - 1;
-                    bool eum = u % 16 == 0, eup = (u % 16 == 15) || (u == (ax == 0 ? g.ny : g.nx) - 1);
-                    int nv = ax == 2 ? g.ny : g.nz;
-                    bool evm = v % 16 == 0, evp = (v % 16 == 15) || (v == nv - 1);
-                    for (int l = 0; l < 3; ++l) {
-                        xg[l] = xd(p, l, pt);
-                        xin[l] = xd(qin, l, pin);
-                        um[l] = eum ? 0 : xd(qum, l, pum);
-                        up[l] = eup ? 0 : xd(qup, l, pup);
-                        vm[l] = evm ? 0 : xd(qvm, l, pvm);
-                        vp[l] = evp ? 0 : xd(qvp, l, pvp);
-                        b0[l] = B0[l * P + a];
-                    }
-                    ghost_b(g, ax, side, u, v, xg, xin, um, up, vm, vp, b0, Bd, det);
-                    for (int l = 0; l < 3; ++l) Bv[l] = (float)Bd[l];
-                    if (A.emit && owns(g, gi, gj, p, ix0, ixe, iy0, iye, Ks, Ke)) {
-                        for (int l = 0; l < 3; ++l) A.eB32[l * P + a] = Bv[l];
-                        A.edet[a] = det;
-                    }
-                }
-                for (int l = 0; l < 3; ++l) BL[(sb * 3 + l) * NPTS + pt] = Bv[l];
-            }
-        }
-        __syncthreads();
-
-        const int pj = p - 1;
-        if (pj >= Ks && pj <= Ke) {
-            for (int pt = threadIdx.x; pt < npts; pt += NT) {
-                int c = pt % RW, r = pt / RW;
-                int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-                if (c < 2 || c > RW - 3 || r < 2 || r > RH - 3) continue;
-                if (gi < 1 || gi > ixe || gj < 1 || gj > iye) continue;
-                long long a = gidx(g, gi, gj, pj);
-                const int nb4[3][4][2] = {{{0, 0}, {1, 0}, {2, 0}, {3, 0}},
-                                          {{1, 0}, {4, -1}, {3, 0}, {5, 1}},
-                                          {{4, -1}, {0, 0}, {5, 1}, {2, 0}}};
-                const int off[6] = {-1, -RW, +1, +RW, 0, 0};
-                float Bc[3];
-                for (int l = 0; l < 3; ++l) Bc[l] = BLv(pj, l, pt);
-                float NA[3][3], II[3];
-                for (int q = 0; q < 3; ++q) {
-                    int pv[4], qv[4];
-                    for (int e = 0; e < 4; ++e) {
-                        pv[e] = pt + off[nb4[q][e][0]];
-                        qv[e] = pj + nb4[q][e][1];
-                    }
-                    float d[4][3], bv[4][3];
-                    for (int l = 0; l < 3; ++l) {
-                        for (int e = 0; e < 4; ++e) {
-                            int e1 = (e + 1) & 3;
-                            d[e][l] = dif(qv[e1], pv[e1], qv[e], pv[e], l);
-                            // centred on the point's own field: exact in real
-                            // arithmetic because the edges close
-                            bv[e][l] = BLv(qv[e], l, pv[e]) - Bc[l];
-                        }
-                    }
-                    II[q] = diamond(d, bv, NA[q]);
-                }
-                float Jv[3], kv[3];
-                solve_na(NA, II, Jv);
-                cross3(Jv, Bc, kv);
-                int ii = gi - 1, jj = gj - 1, kk = pj - 1;
-                float eps = g.epsilonProf ? (float)(A.epsx[ii] * A.epsy[jj] * A.epsz[kk]) : 1.0f;
-                float gv[3] = {0, 0, 0};
-                double detv = 0;
-                if (g.pressure || A.emit || A.diag) {
-                    float jac[3][3], d[3];
-                    for (int l = 0; l < 3; ++l) d[l] = dif(pj, pt + 1, pj, pt - 1, l);
-                    jac_col(jac, 0, d, h1x);
-                    for (int l = 0; l < 3; ++l) d[l] = dif(pj, pt + RW, pj, pt - RW, l);
-                    jac_col(jac, 1, d, h1y);
-                    for (int l = 0; l < 3; ++l) d[l] = dif(pj + 1, pt, pj - 1, pt, l);
-                    jac_col(jac, 2, d, h1z);
-                    double jd[3][3];
-                    for (int u = 0; u < 3; ++u)
-                        for (int v = 0; v < 3; ++v) jd[u][v] = jac[u][v];
-                    detv = det3(jd);
-                    if (g.pressure) {
-                        // second differences as differences of differences
-                        auto dd2 = [&](int l, int dia, int dja, int dka, int dib, int djb, int dkb) {
-                            return dif(pj + dka, pt + dia + dja * RW, pj + dkb, pt + dib + djb * RW, l);
-                        };
-                        float d2[3][3][3];
-                        for (int l = 0; l < 3; ++l) {
-                            d2[l][0][0] = (dd2(l, 1, 0, 0, 0, 0, 0) - dd2(l, 0, 0, 0, -1, 0, 0)) * h1x * h1x;
-                            d2[l][1][1] = (dd2(l, 0, 1, 0, 0, 0, 0) - dd2(l, 0, 0, 0, 0, -1, 0)) * h1y * h1y;
-                            d2[l][2][2] = (dd2(l, 0, 0, 1, 0, 0, 0) - dd2(l, 0, 0, 0, 0, 0, -1)) * h1z * h1z;
-                            d2[l][0][1] = (dd2(l, 1, 1, 0, 1, -1, 0) - dd2(l, -1, 1, 0, -1, -1, 0)) * h1x * h1y / 4;
-                            d2[l][0][2] = (dd2(l, 1, 0, 1, 1, 0, -1) - dd2(l, -1, 0, 1, -1, 0, -1)) * h1x * h1z / 4;
-                            d2[l][1][2] = (dd2(l, 0, 1, 1, 0, -1, 1) - dd2(l, 0, 1, -1, 0, -1, -1)) * h1y * h1z / 4;
-                            d2[l][1][0] = d2[l][0][1];
-                            d2[l][2][0] = d2[l][0][2];
-                            d2[l][2][1] = d2[l][1][2];
-                        }
-                        gradp_core(jac, d2, (float)(1.0 / detv), (float)g.beta, gv);
-                    }
-                }
+            } else {
+                // x_6 = x_5 + sum_m (b6m - b5m) k_m, then the x5/x4 weights
+                // (src/rungeKutta.cu:216-227, 296-309); x_new to buffer cur+1,
+                // x_6 (where a rejected attempt leaves xb) to cur+2
+                double *Xd = A.X64[(cur + 1) % 3], *X6 = A.X64[(cur + 2) % 3];
+#pragma unroll
                 for (int l = 0; l < 3; ++l) {
-                    kv[l] *= dt * eps;
-                    if (g.pressure) kv[l] += -dt * gv[l];
-                }
-                if (A.emit) {
-                    for (int l = 0; l < 3; ++l) {
-                        A.eB32[l * P + a] = Bc[l];
-                        A.eJ32[l * P + a] = Jv[l];
-                        if (g.pressure) A.eG32[l * P + a] = gv[l];
-                    }
-                    A.edet[a] = detv;
-                }
-                if (A.diag) {
-                    double Bd[3] = {Bc[0], Bc[1], Bc[2]}, Jd[3] = {Jv[0], Jv[1], Jv[2]}, jxb[3];
-                    cross3(Jd, Bd, jxb);
-                    double b2 = Bd[0] * Bd[0] + Bd[1] * Bd[1] + Bd[2] * Bd[2];
-                    double nc = sqrt(jxb[0] * jxb[0] + jxb[1] * jxb[1] + jxb[2] * jxb[2]);
-                    double nj = sqrt(Jd[0] * Jd[0] + Jd[1] * Jd[1] + Jd[2] * Jd[2]);
-                    double v = b2 * detv;
-                    int bound = (ii == 0) + (jj == 0) + (kk == 0) + (ii == g.nx - 1) + (jj == g.ny - 1) + (kk == g.nz - 1);
-                    v /= (1 << bound);
-                    e_sum += v;
-                    jmax = nb_max(jmax, nj);
-                    r1 = nb_max(r1, nc / b2);
-                    double den = nj * sqrt(b2);
-                    r2 = nb_max(r2, den > 0 ? nc / den : 0.0);
-                }
-                if (stage >= 0 && stage < 5) {
-                    for (int l = 0; l < 3; ++l) A.K32[stage * P3 + l * P + a] = kv[l];
-                } else if (stage == 5) {
-                    float *Xhd = A.Xh[cur ^ 1], *Xld = A.Xl[cur ^ 1];
-                    for (int l = 0; l < 3; ++l) {
-                        float hi = XHg[l * P + a], lo = XLg[l * P + a];
-                        if (l < 2 || g.zUpdate) {
-                            float k6[6];
-                            for (int m = 0; m < 5; ++m) k6[m] = K[m * P3 + l * P + a];
-                            k6[5] = kv[l];
-                            float dC = 0;
-                            double err = 0;
-                            for (int m = 0; m < 6; ++m) {
-                                dC += (float)A.tab.c[m] * k6[m];
-                                err += A.tab.ce[m] * (double)k6[m];
-                            }
-                            // (hi, lo) + dC with the rounding error of hi + dC kept
-                            float s, e;
-                            two_sum(hi, dC, s, e);
-                            float lo2 = lo + e;
-                            float hi2 = s + lo2;
-                            lo = lo2 - (hi2 - s);
-                            hi = hi2;
-                            double dd = fabs(err);
-                            if (dd != dd) nan_seen = 1;
-                            dmax = nb_max(dmax, dd);
+                    double k6[6];
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) k6[m] = A.K64[m * P3 + l * P + a];
+                    k6[5] = kv[l];
+                    double x6 = xq[1].x[l];
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) x6 += c_tab.db[5 * 6 + m] * k6[m];
+                    X6[l * P + a] = x6;
+                    double xn = x6, xs = x6;
+                    if (l < 2 || g.zUpdate)
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) {
+                            xn += c_tab.c[m] * k6[m];
+                            xs += c_tab.cs[m] * k6[m];
                         }
-                        Xhd[l * P + a] = hi;
-                        Xld[l * P + a] = lo;
-                    }
+                    Xd[l * P + a] = xn;
+                    double dd = fabs(xn - xs);
+                    if (dd != dd) nan_seen = 1;
+                    dmax = nb_max(dmax, dd);
                 }
             }
         }
-        __syncthreads();
+
+        // rotate the column queues
+        xq[0] = xq[1];
+        xq[1] = xq[2];
+        xq[2] = xq[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            bq[0][l] = bq[1][l];
+            bq[1][l] = bq[2][l];
+            b0q[0][l] = b0q[1][l];
+        }
+        dq[0] = dq[1];
+        if constexpr (FINAL && F32) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                fl[0][l] = fl[1][l]; fl[1][l] = fl[2][l];
+                fc[0][l] = fc[1][l]; fc[1][l] = fc[2][l];
+                fe[0][l] = fe[1][l]; fe[1][l] = fe[2][l];
+            }
+        }
     }
 
+    // ---- block partials (fixed order: deterministic)
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (stage == 5) {
+    if (FINAL) {
         dmax = wave_max(dmax);
         if (lane == 0) red[0][w] = dmax;
         if (nan_seen) atomicOr(&ctl->nan_seen, 1);
@@ -739,7 +757,7 @@ __global__ void __launch_bounds__(NT) k_fused_f32k(FusedArgs A)
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (stage == 5) {
+        if (FINAL) {
             double m = 0;
             for (int q = 0; q < NT / 64; ++q) m = nb_max(m, red[0][q]);
             A.part_delta[blockIdx.x] = m;
@@ -852,39 +870,80 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
 }
 
 // ================================================================ launch
-static bool g_attr_done[8];
+namespace {
+constexpr int kNT32 = 512;   // F32K: threads (= largest rect) per workgroup
+constexpr int kNT64 = 512;   // F64
 
-template <int NT, int NPTS>
-static int launch_fused_t(hipStream_t s, const FusedArgs &A, int f32k)
+template <bool F32, int NT>
+size_t lds_bytes()
 {
-    size_t lds = f32k ? (size_t)(4 * 3 * NPTS * 2 + 3 * 3 * NPTS) * sizeof(float)
-                      : (size_t)(4 * 3 * NPTS + 3 * 3 * NPTS) * sizeof(double);
-    if (f32k) {
-        if (!g_attr_done[1]) {
-            (void)hipFuncSetAttribute((const void *)k_fused_f32k<NT, NPTS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            g_attr_done[1] = true;
-        }
-        hipLaunchKernelGGL((k_fused_f32k<NT, NPTS>), dim3(A.nblocks), dim3(NT), lds, s, A);
-    } else {
-        if (!g_attr_done[0]) {
-            (void)hipFuncSetAttribute((const void *)k_fused_f64<NT, NPTS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            g_attr_done[0] = true;
-        }
-        hipLaunchKernelGGL((k_fused_f64<NT, NPTS>), dim3(A.nblocks), dim3(NT), lds, s, A);
-    }
-    return 0;
+    return (size_t)((F32 ? 2 : 1) * 5 * 3 * NT + 3 * 3 * NT) * (F32 ? sizeof(float) : sizeof(double));
 }
 
-int fused_max_points() { return kFusedPts; }
-size_t fused_lds_bytes(int f32k)
+template <bool F32, int NT, int STAGE, bool PRESS>
+void launch_one(hipStream_t s, const FusedArgs &A)
 {
-    return f32k ? (size_t)(4 * 3 * kFusedPts * 2 + 3 * 3 * kFusedPts) * sizeof(float)
-                : (size_t)(4 * 3 * kFusedPts + 3 * 3 * kFusedPts) * sizeof(double);
+    static bool attr = false;
+    size_t lds = lds_bytes<F32, NT>();
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_stage<F32, NT, STAGE, PRESS>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_stage<F32, NT, STAGE, PRESS>), dim3(A.nblocks), dim3(NT), lds, s, A);
+}
+
+template <bool F32, int NT, bool PRESS>
+void launch_stage(hipStream_t s, const FusedArgs &A)
+{
+    switch (A.stage) {
+    case 0: launch_one<F32, NT, 0, PRESS>(s, A); break;
+    case 1: launch_one<F32, NT, 1, PRESS>(s, A); break;
+    case 2: launch_one<F32, NT, 2, PRESS>(s, A); break;
+    case 3: launch_one<F32, NT, 3, PRESS>(s, A); break;
+    case 4: launch_one<F32, NT, 4, PRESS>(s, A); break;
+    case 5: launch_one<F32, NT, 5, PRESS>(s, A); break;
+    default: launch_one<F32, NT, 6, PRESS>(s, A); break;
+    }
+}
+}  // namespace
+
+int fused_max_points(int f32k) { return f32k ? kNT32 : kNT64; }
+
+void set_tableau(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof t); }
+
+long long ghost_points(const Geo &g, long long off[6])
+{
+    long long tot = 0;
+    for (int fs = 0; fs < 6; ++fs) {
+        int f = fs >> 1;
+        off[fs] = tot;
+        tot += f == 0 ? (long long)g.ny * g.nz : f == 1 ? (long long)g.nx * g.nz : (long long)g.nx * g.ny;
+    }
+    return tot;
+}
+
+void launch_ghost_setup(hipStream_t s, const FusedArgs &A, int f32k, GhostPt *out)
+{
+    long long off[6];
+    long long tot = ghost_points(A.g, off);
+    unsigned nb = (unsigned)((tot + 255) / 256);
+    if (f32k)
+        hipLaunchKernelGGL(k_ghost_setup<true>, dim3(nb), dim3(256), 0, s, A, out);
+    else
+        hipLaunchKernelGGL(k_ghost_setup<false>, dim3(nb), dim3(256), 0, s, A, out);
 }
 
 void launch_fused(hipStream_t s, const FusedArgs &A, int f32k)
 {
-    launch_fused_t<kFusedThreads, kFusedPts>(s, A, f32k);
+    bool pr = A.g.pressure != 0;
+    if (f32k) {
+        if (pr) launch_stage<true, kNT32, true>(s, A);
+        else launch_stage<true, kNT32, false>(s, A);
+    } else {
+        if (pr) launch_stage<false, kNT64, true>(s, A);
+        else launch_stage<false, kNT64, false>(s, A);
+    }
 }
 
 void launch_controller(hipStream_t s, const CtlArgs &A)

@@ -45,6 +45,8 @@ struct gm_ctx {
     int part_cap = 0;
     gmd::Tableau *tab_dev = nullptr;
     gmd::TableauF tabf{};
+    gmd::GhostPt *ghost = nullptr;
+    long long ghost_off[6] = {0, 0, 0, 0, 0, 0};
     gmd::StepRecord *log = nullptr;
     int log_cap = 4096;
     double *red_out = nullptr, *red_host = nullptr;   // 4 doubles each
@@ -129,7 +131,7 @@ bool any_peri(const gm_params &p) { return p.xPeri || p.yPeri || p.zPeri; }
 // split z so that the launch has about `target` workgroups.
 void auto_tiles(gm_ctx *c, int target)
 {
-    const int cap = gmd::fused_max_points();
+    const int cap = gmd::fused_max_points(c->prec == GM_PREC_F32K);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     double best = 1e30;
     int bx = 0, by = 0;
@@ -168,7 +170,6 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
 {
     gmd::FusedArgs A{};
     A.g = c->geo;
-    A.tab = c->tabf;
     A.stage = stage;
     A.TX = c->TX; A.TY = c->TY; A.CZ = c->CZ;
     A.tiles_x = c->tiles_x; A.tiles_y = c->tiles_y; A.nchunks = c->nchunks; A.nblocks = c->nblocks;
@@ -181,6 +182,8 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
     A.eB32 = c->B32; A.eJ32 = c->J32; A.eG32 = c->G32;
     A.edet = c->det;
     A.epsx = c->epsx; A.epsy = c->epsy; A.epsz = c->epsz;
+    A.ghost = c->ghost;
+    for (int i = 0; i < 6; ++i) A.ghost_off[i] = c->ghost_off[i];
     A.part_delta = c->part_delta; A.part_diag = c->part_diag;
     A.ctl = c->ctl;
     return A;
@@ -341,6 +344,8 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     for (int m = 0; m < 6; ++m) c->tabf.ce[m] = tab.c[m] - tab.cs[m];
     if ((r = dalloc(&c->tab_dev, 1))) return bail(r);
     (void)hipMemcpy(c->tab_dev, &tab, sizeof tab, hipMemcpyHostToDevice);
+    gmd::set_tableau(c->tabf);
+    if ((r = dalloc(&c->ghost, (size_t)gmd::ghost_points(c->geo, c->ghost_off)))) return bail(r);
     // control block, log, reductions
     if ((r = dalloc(&c->ctl, 1))) return bail(r);
     if (hipHostMalloc((void **)&c->ctlh, sizeof(gmd::Ctl), hipHostMallocDefault) != hipSuccess)
@@ -372,7 +377,7 @@ extern "C" int gm_destroy(gm_ctx *c)
     for (int i = 0; i < 2; ++i) { dfree(c->Xh[i]); dfree(c->Xl[i]); }
     dfree(c->B0_32); dfree(c->K32); dfree(c->B32); dfree(c->J32); dfree(c->G32);
     dfree(c->det); dfree(c->epsx); dfree(c->epsy); dfree(c->epsz);
-    dfree(c->ctl); dfree(c->part_delta); dfree(c->part_diag); dfree(c->tab_dev); dfree(c->log);
+    dfree(c->ctl); dfree(c->part_delta); dfree(c->part_diag); dfree(c->tab_dev); dfree(c->log); dfree(c->ghost);
     dfree(c->red_out); dfree(c->diag_scratch); dfree(c->nanflag);
     if (c->ctlh) (void)hipHostFree(c->ctlh);
     if (c->red_host) (void)hipHostFree(c->red_host);
@@ -415,7 +420,13 @@ extern "C" int gm_upload(gm_ctx *c, const double *B0, const double *xb, double t
     h.n_cycle = 1LL << 62;
     h.max_attempt = 1LL << 62;
     c->cur = 0;
-    return push_ctl(c);
+    int r2 = push_ctl(c);
+    if (r2) return r2;
+    // constants of the line-tied ghost faces (the ghost layer never moves)
+    gmd::FusedArgs A = fused_args(c, 0, 0, 0, 0);
+    gmd::launch_ghost_setup(c->stream, A, c->prec == GM_PREC_F32K, c->ghost);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GM_OK;
 }
 
 extern "C" int gm_get_time(gm_ctx *c, double *t, double *dt)
@@ -788,7 +799,7 @@ extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
         auto_tiles(c, cz > 0 ? cz : 512);
         return ensure_partials(c);
     }
-    if ((tx + 4) * (ty + 4) > gmd::fused_max_points())
+    if ((tx + 4) * (ty + 4) > gmd::fused_max_points(c->prec == GM_PREC_F32K))
         return fail(GM_ERR_ARG, "tile rect (tx+4)(ty+4) exceeds the compiled capacity");
     c->TX = std::min(tx, c->p.nx);
     c->TY = std::min(ty, c->p.ny);
