@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is design only "
@@ -109,6 +111,11 @@ def main():
 
     import glemur_amd as g
     from glemur_amd import workloads
+
+    copy_gbs = None
+    if args.copy_calibration:
+        ms = g.copy_probe(1 << 30, 5)
+        copy_gbs = 2 * (1 << 30) / (ms * 1e-3) / 1e9
 
     p = workloads.medium()
     B0, xb = g.init_state(p)
@@ -147,6 +154,8 @@ def main():
                      "mean_launch_ms_final": f32["ms_stage_final"]},
         "grid_pt_stages_per_s_hbm_pct": 100 * achieved / HBM_PEAK_GBS,
     }
+    if copy_gbs is not None:
+        out["copy_probe_GBs"] = copy_gbs
     if not args.no_f64:
         f64 = measure(g, p, B0, xb, g.PREC_F64, max(50, args.steps // 3), args.warmup, dt0)
         b64 = g.algorithmic_bytes(g.PREC_F64) * n_pts

@@ -146,6 +146,7 @@ _SIGS = {
     "gm_timing_read": (C.c_int, [_CTX, _D, _P(C.c_int64)]),
     "gm_algorithmic_bytes_per_point_stage": (C.c_double, [C.c_int]),
     "gm_set_tile": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_int]),
+    "gm_copy_probe": (C.c_int, [C.c_int64, C.c_int, _D]),
     "gm_get_tile": (C.c_int, [_CTX, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "gm_write_b0": (C.c_int, [C.c_char_p, _P(Params), _D, C.c_int]),
     "gm_dump_state": (C.c_int, [C.c_char_p, _P(Params), C.c_double, C.c_double, C.c_int,
@@ -213,6 +214,13 @@ def init_residual(p):
 def braid_seeded(p, seed, nmin=6, nmax=6, random_centres=False):
     _check(_lib.gm_braid_seeded(C.byref(p), C.c_uint64(seed), nmin, nmax, int(random_centres)))
     return p
+
+
+def copy_probe(nbytes, reps=10):
+    """Mean ms of a plain device copy of nbytes (reads nbytes, writes nbytes)."""
+    ms = C.c_double()
+    _check(_lib.gm_copy_probe(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def algorithmic_bytes(precision):

@@ -317,7 +317,8 @@ __global__ void __launch_bounds__(NT) k_stage(FusedArgs A)
 
     auto plane_ok = [&](int q) { return q >= max(Ks - 2, 0) && q <= min(Ke + 2, g.nz + 1); };
     auto ghost_pt = [&](int q) { return gx || gy || q == 0 || q == g.nz + 1; };
-    auto XLslot = [&](int q) { return ((q % NXL) + NXL) % NXL; };
+    // ring slot of plane q (q > -NXL * 4 always holds here)
+    auto XLslot = [&](int q) { return (unsigned)(q + NXL * 4) % (unsigned)NXL; };
     auto ld_x = [&](int q, int pt, Pos &o) {
         int s = XLslot(q);
 #pragma unroll
@@ -343,7 +344,7 @@ __global__ void __launch_bounds__(NT) k_stage(FusedArgs A)
         }
     };
     auto ld_b = [&](int q, int pt, R o[3]) {
-        int s = ((q % 3) + 3) % 3;
+        int s = (unsigned)(q + 12) % 3u;
 #pragma unroll
         for (int l = 0; l < 3; ++l) o[l] = BL[(s * 3 + l) * NT + pt];
     };
@@ -495,7 +496,7 @@ __global__ void __launch_bounds__(NT) k_stage(FusedArgs A)
                     }
                 }
             }
-            int sb = ((p % 3) + 3) % 3;
+            int sb = (unsigned)(p + 12) % 3u;
 #pragma unroll
             for (int l = 0; l < 3; ++l) {
                 bq[2][l] = Bv[l];
@@ -503,9 +504,10 @@ __global__ void __launch_bounds__(NT) k_stage(FusedArgs A)
             }
             dq[1] = detv;
         }
-        __syncthreads();
 
-        // (3) J, force and slope at (column, p-1)
+        // (3) J, force and slope at (column, p-1): in-plane data of plane p-1
+        // was published before the previous iteration's barrier, B(p) is this
+        // thread's own
         const int pj = p - 1;
         if (inJ && pj >= Ks && pj <= Ke) {
             const long long a = abase + g.sxy * (long long)pj;
@@ -714,6 +716,10 @@ __global__ void __launch_bounds__(NT) k_stage(FusedArgs A)
                 }
             }
         }
+
+        // one barrier per plane: LDS writes of this iteration become visible,
+        // and no slot still read here is overwritten before everyone passed
+        __syncthreads();
 
         // rotate the column queues
         xq[0] = xq[1];

@@ -503,6 +503,13 @@ __global__ void k_padded_to_inner_aos(Geo g, const R *src, double *dst)
     long long a = gidx(g, i + 1, j + 1, k + 1);
     for (int l = 0; l < 3; ++l) dst[3 * t + l] = (double)src[l * g.P + a];
 }
+// Plain device copy: the reference kernel for HBM byte counters and the
+// achievable-bandwidth probe.
+__global__ void __launch_bounds__(256) k_copy_probe(const float4 *src, float4 *dst, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 __global__ void k_fill_f64(double *p, long long n, double v)
 {
     long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -610,6 +617,11 @@ void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, doubl
 }
 template void launch_padded_to_inner_aos<double>(hipStream_t, const Geo &, const double *, double *);
 template void launch_padded_to_inner_aos<float>(hipStream_t, const Geo &, const float *, double *);
+void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long bytes)
+{
+    long long n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(256 * 32), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n);
+}
 void launch_fill_f64(hipStream_t s, double *p, long long n, double v)
 {
     hipLaunchKernelGGL(k_fill_f64, dim3(blocks_for(n)), dim3(kBlock), 0, s, p, n, v);

@@ -56,6 +56,7 @@ struct gm_ctx {
     int cur = 0;                  // host copy of ctl->cur (valid after syncs)
     // tiles
     int TX = 0, TY = 0, CZ = 0, tiles_x = 0, tiles_y = 0, nchunks = 0, nblocks = 0;
+    int ncu = 256;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -133,23 +134,31 @@ void auto_tiles(gm_ctx *c, int target)
 {
     const int cap = gmd::fused_max_points(c->prec == GM_PREC_F32K);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
-    double best = 1e30;
-    int bx = 0, by = 0;
-    for (int ty = 2; ty <= 32; ++ty)
-        for (int txm = 4; (txm + 4) * (ty + 4) <= cap; ++txm) {
-            int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
-            int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
-            if ((tx + 4) * (tyy + 4) > cap) continue;
-            double loaded = (double)tlx * tly * (tx + 4) * (tyy + 4);
-            double cost = loaded / ((double)nx * ny);
-            if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
-        }
+    // Preferred: a 32-point wide rect (tile 28 x (cap/32 - 4)), so that
+    // lanes map to x in half-waves, the two top and bottom rows form whole
+    // waves that skip the current evaluation, and rows are 128 B of FP32.
+    int bx = 28, by = cap / 32 - 4;
+    if (nx < bx || ny < by) {
+        // small grids: smallest loaded-points-per-output-point rect that fits
+        double best = 1e30;
+        for (int ty = 2; ty <= 32; ++ty)
+            for (int txm = 4; (txm + 4) * (ty + 4) <= cap; ++txm) {
+                int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
+                int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
+                if ((tx + 4) * (tyy + 4) > cap) continue;
+                double cost = (double)tlx * tly * (tx + 4) * (tyy + 4) / ((double)nx * ny);
+                if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
+            }
+    }
     c->TX = bx;
     c->TY = by;
     c->tiles_x = (nx + bx - 1) / bx;
     c->tiles_y = (ny + by - 1) / by;
     int cols = c->tiles_x * c->tiles_y;
-    int nch = std::max(1, std::min(nz / 8 > 0 ? nz / 8 : 1, (target + cols - 1) / cols));
+    // z chunks: as many workgroups as the CUs hold in one or more whole
+    // rounds (target = CUs x rounds), never chunks thinner than 8 planes
+    int nch = std::max(1, target / cols);
+    nch = std::min(nch, std::max(1, nz / 8));
     c->CZ = (nz + nch - 1) / nch;
     c->nchunks = (nz + c->CZ - 1) / c->CZ;
     c->nblocks = cols * c->nchunks;
@@ -360,7 +369,11 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
         return bail(r);
     if (hipHostMalloc((void **)&c->red_host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(GM_ERR_HIP, "hipHostMalloc"));
-    auto_tiles(c, 512);
+    {
+        hipDeviceProp_t pr;
+        c->ncu = hipGetDeviceProperties(&pr, c->device) == hipSuccess ? pr.multiProcessorCount : 256;
+    }
+    auto_tiles(c, c->ncu);
     if ((r = ensure_partials(c))) return bail(r);
     if ((r = push_ctl(c))) return bail(r);
     *out = c;
@@ -796,7 +809,7 @@ extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     if (tx <= 0 || ty <= 0) {
-        auto_tiles(c, cz > 0 ? cz : 512);
+        auto_tiles(c, cz > 0 ? cz : c->ncu);
         return ensure_partials(c);
     }
     if ((tx + 4) * (ty + 4) > gmd::fused_max_points(c->prec == GM_PREC_F32K))
@@ -818,6 +831,35 @@ extern "C" int gm_get_tile(gm_ctx *c, int *tx, int *ty, int *cz, int *nblocks)
     if (ty) *ty = c->TY;
     if (cz) *cz = c->CZ;
     if (nblocks) *nblocks = c->nblocks;
+    return GM_OK;
+}
+
+// Device-to-device copy of `bytes` with a plain kernel, `reps` times; the
+// mean kernel time in ms.  No reference counterpart: the yardstick for HBM
+// byte counters and achievable bandwidth.
+extern "C" int gm_copy_probe(int64_t bytes, int reps, double *ms_mean)
+{
+    if (bytes < 16 || reps < 1) return fail(GM_ERR_ARG, "gm_copy_probe: bad size");
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, (size_t)bytes));
+    HIPCHK(hipMalloc(&b, (size_t)bytes));
+    (void)hipMemset(a, 0, (size_t)bytes);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    gmd::launch_copy_probe(nullptr, a, b, bytes);
+    (void)hipEventRecord(e0, nullptr);
+    for (int i = 0; i < reps; ++i) gmd::launch_copy_probe(nullptr, a, b, bytes);
+    (void)hipEventRecord(e1, nullptr);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(GM_ERR_HIP, hipGetErrorString(e));
+    if (ms_mean) *ms_mean = ms / reps;
     return GM_OK;
 }
 

@@ -230,7 +230,11 @@ int gm_timing_enable(gm_ctx *ctx, int on);
 int gm_timing_read(gm_ctx *ctx, double ms[3], int64_t count[3]);
 /* Bytes per grid point and stage the fused kernel must move (algorithmic). */
 double gm_algorithmic_bytes_per_point_stage(int precision);
+/* Plain device copy of `bytes`, `reps` times: mean kernel ms (the yardstick
+ * for HBM byte counters; no reference counterpart). */
+int gm_copy_probe(int64_t bytes, int reps, double *ms_mean);
 /* Fused-kernel tile configuration (x, y tile and z chunk); 0 = automatic. */
+/* tx or ty <= 0 picks the tile automatically for about `cz` workgroups. */
 int gm_set_tile(gm_ctx *ctx, int tx, int ty, int cz);
 int gm_get_tile(gm_ctx *ctx, int *tx, int *ty, int *cz, int *nblocks);
 
