@@ -241,12 +241,21 @@ __device__ __forceinline__ void stokes_f32(const float r[6][3], const float b[6]
 }
 }  // namespace
 
+// Waves per SIMD a stage kernel is register-allocated for: the FP32 stages
+// without pressure fit 128 VGPRs (two 512-thread workgroups per CU), the
+// FP64 stages, the pressure variants and the FP32 final stage need more.
+constexpr int stage_waves(bool f32, int stage, bool press)
+{
+    return (f32 && !press && stage != 5) ? 4 : 2;
+}
+
 // =============================================================== kernel
 // F32: FP32 stages with compensated positions (else FP64).  NT: threads =
 // largest rect.  STAGE: 0..5, or 6 for the fields of the current positions
 // only (no slope written).  PRESS: pressure force on.
 template <bool F32, int NT, int STAGE, bool PRESS>
-__global__ void __launch_bounds__(NT) k_stage(FusedArgs A)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(stage_waves(F32, STAGE, PRESS), 8)))
+k_stage(FusedArgs A)
 {
     constexpr bool FINAL = STAGE == 5;
     using PT = PrecT<F32>;

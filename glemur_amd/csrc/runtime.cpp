@@ -134,22 +134,18 @@ void auto_tiles(gm_ctx *c, int target)
 {
     const int cap = gmd::fused_max_points(c->prec == GM_PREC_F32K);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
-    // Preferred: a 32-point wide rect (tile 28 x (cap/32 - 4)), so that
-    // lanes map to x in half-waves, the two top and bottom rows form whole
-    // waves that skip the current evaluation, and rows are 128 B of FP32.
-    int bx = 28, by = cap / 32 - 4;
-    if (nx < bx || ny < by) {
-        // small grids: smallest loaded-points-per-output-point rect that fits
-        double best = 1e30;
-        for (int ty = 2; ty <= 32; ++ty)
-            for (int txm = 4; (txm + 4) * (ty + 4) <= cap; ++txm) {
-                int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
-                int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
-                if ((tx + 4) * (tyy + 4) > cap) continue;
-                double cost = (double)tlx * tly * (tx + 4) * (tyy + 4) / ((double)nx * ny);
-                if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
-            }
-    }
+    // smallest loaded-points-per-output-point rect (tile plus its 2-point
+    // halo) that fits one workgroup
+    int bx = 4, by = 2;
+    double best = 1e30;
+    for (int ty = 2; ty <= 64; ++ty)
+        for (int txm = 4; (txm + 4) * (ty + 4) <= cap; ++txm) {
+            int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
+            int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
+            if ((tx + 4) * (tyy + 4) > cap) continue;
+            double cost = (double)tlx * tly * (tx + 4) * (tyy + 4) / ((double)nx * ny);
+            if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
+        }
     c->TX = bx;
     c->TY = by;
     c->tiles_x = (nx + bx - 1) / bx;
@@ -162,6 +158,13 @@ void auto_tiles(gm_ctx *c, int target)
     c->CZ = (nz + nch - 1) / nch;
     c->nchunks = (nz + c->CZ - 1) / c->CZ;
     c->nblocks = cols * c->nchunks;
+}
+
+// Workgroups per launch: one round over the CUs at the stage kernels'
+// occupancy (two 512-thread workgroups per CU for the FP32 stages).
+int default_blocks(const gm_ctx *c)
+{
+    return c->ncu * (c->prec == GM_PREC_F32K ? 2 : 1);
 }
 
 int ensure_partials(gm_ctx *c)
@@ -373,7 +376,7 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
         hipDeviceProp_t pr;
         c->ncu = hipGetDeviceProperties(&pr, c->device) == hipSuccess ? pr.multiProcessorCount : 256;
     }
-    auto_tiles(c, c->ncu);
+    auto_tiles(c, default_blocks(c));
     if ((r = ensure_partials(c))) return bail(r);
     if ((r = push_ctl(c))) return bail(r);
     *out = c;
@@ -809,7 +812,7 @@ extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     if (tx <= 0 || ty <= 0) {
-        auto_tiles(c, cz > 0 ? cz : c->ncu);
+        auto_tiles(c, cz > 0 ? cz : default_blocks(c));
         return ensure_partials(c);
     }
     if ((tx + 4) * (ty + 4) > gmd::fused_max_points(c->prec == GM_PREC_F32K))
