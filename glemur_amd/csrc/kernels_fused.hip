@@ -242,11 +242,12 @@ __device__ __forceinline__ void stokes_f32(const float r[6][3], const float b[6]
 }  // namespace
 
 // Waves per SIMD a stage kernel is register-allocated for: the FP32 stages
-// without pressure fit 128 VGPRs (two 512-thread workgroups per CU), the
-// FP64 stages, the pressure variants and the FP32 final stage need more.
+// 0-2 and the fields-only pass fit 128 VGPRs without spilling (two
+// 512-thread workgroups per CU); the later FP32 stages hold more slopes in
+// flight and, like the FP64 and pressure variants, keep 256 VGPRs.
 constexpr int stage_waves(bool f32, int stage, bool press)
 {
-    return (f32 && !press && stage != 5) ? 4 : 2;
+    return (f32 && !press && (stage <= 2 || stage == 6)) ? 4 : 2;
 }
 
 // =============================================================== kernel
