@@ -82,7 +82,7 @@ class Params(C.Structure):
 class Reduction(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "B2", "B2res", "BMax", "JMax", "JxB_B2Max", "epsilonStar", "errB_1ez", "errXb_XbAn",
-        "convex", "wedgeMin", "U2", "UMax", "JxB_JBMax")]
+        "convex", "wedgeMin", "U2", "UMax", "JxB_JBMax", "JxBMax_JBMax")]
 
 
 class StepInfo(C.Structure):
@@ -98,13 +98,14 @@ class RunOpts(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("n_attempt", C.c_int64), ("n_accept", C.c_int64), ("t", C.c_double), ("dt", C.c_double),
-                ("ratio", C.c_double), ("energy", C.c_double), ("converged", C.c_int), ("stopped", C.c_int),
+                ("ratio", C.c_double), ("ratio_pt", C.c_double), ("energy", C.c_double),
+                ("converged", C.c_int), ("stopped", C.c_int),
                 ("nan_seen", C.c_int), ("seconds", C.c_double)]
 
 
 class StepRecord(C.Structure):
     _fields_ = [("t", C.c_double), ("dt_used", C.c_double), ("maxDelta", C.c_double), ("ratio", C.c_double),
-                ("energy", C.c_double), ("jxb_b2", C.c_double), ("jmax", C.c_double),
+                ("ratio_pt", C.c_double), ("energy", C.c_double), ("jxb_b2", C.c_double), ("jmax", C.c_double),
                 ("accepted", C.c_int32), ("nan_seen", C.c_int32)]
 
 

@@ -32,6 +32,7 @@ struct Ctl {
     double t, dt;           // time and dt of the next attempt
     double dt_used, maxDelta;
     double ratio, energy, jxb_b2, jmax;  // diagnostics of the last evaluated state
+    double ratio_pt;        // pointwise max |JxB|/(|J||B|) of that state
     int cur;                // index of the current position buffer
     int accepted, nan_seen, done, converged, stop_reason;
     long long n_attempt, n_accept;
@@ -42,7 +43,7 @@ struct Ctl {
 };
 
 struct StepRecord {
-    double t, dt_used, maxDelta, ratio, energy, jxb_b2, jmax;
+    double t, dt_used, maxDelta, ratio, ratio_pt, energy, jxb_b2, jmax;
     int accepted, nan_seen;
 };
 
@@ -62,7 +63,8 @@ int launch_xnewstar(hipStream_t s, const Geo &g, const Tableau *tab, const doubl
 void launch_max_final(hipStream_t s, const double *partials, int n, double *out);
 // Diagnostics over inner points from B, J (padded SoA) and det:
 // out[0] = B2 (incl. dx dy dz), out[1] = JMax, out[2] = JxB_B2Max,
-// out[3] = JxB_JBMax.  `scratch` holds 4*blocks doubles.
+// out[3] = JxB_JBMax, out[4] = max|JxB|, out[5] = max |J||B|.  `scratch`
+// holds 6*blocks doubles.
 template <class R>
 void launch_diag(hipStream_t s, const Geo &g, const R *B, const R *J, const double *det, double *scratch,
                  int max_blocks, double *out);

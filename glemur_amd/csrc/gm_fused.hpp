@@ -43,7 +43,7 @@ struct FusedArgs {
     const GhostPt *ghost;             // [face][side][v][u], offsets in ghost_off
     long long ghost_off[6];
     double *part_delta;               // [nblocks]
-    double *part_diag;                // [4][nblocks]
+    double *part_diag;                // [6][nblocks]: energy, JMax, JxB_B2Max, JxB_JBMax, max|JxB|, max|J||B|
     Ctl *ctl;
 };
 

@@ -270,6 +270,7 @@ extern "C" int gm_write_ts(const char *dir, const gm_params *p, const gm_reducti
         {p->redU2 && p->inertia, "U2", r.U2},
         {p->redUMax && p->inertia, "UMax", r.UMax},
         {jxb_jb_col != 0, "JxB_JBMax", r.JxB_JBMax},
+        {jxb_jb_col != 0, "JxBMax_JBMax", r.JxBMax_JBMax},
     };
     FILE *fd = fopen(fn.c_str(), t == 0 ? "w+" : "a");
     if (!fd) return fail(GM_ERR_IO, "could not open file '" + fn + "'");

@@ -269,7 +269,7 @@ k_stage(FusedArgs A)
     R *XLa = reinterpret_cast<R *>(smem);
     R *XLb = XLa + (F32 ? NXL * 3 * NT : 0);
     R *BL = XLa + (F32 ? 2 : 1) * NXL * 3 * NT;
-    __shared__ double red[5][NT / 64];
+    __shared__ double red[7][NT / 64];
 
     const Geo &g = A.g;
     Ctl *ctl = A.ctl;
@@ -322,7 +322,7 @@ k_stage(FusedArgs A)
     double fe[3][3];
     (void)fl; (void)fc; (void)fe;
 
-    double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0;
+    double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     int nan_seen = 0;
 
     auto plane_ok = [&](int q) { return q >= max(Ks - 2, 0) && q <= min(Ke + 2, g.nz + 1); };
@@ -668,6 +668,8 @@ k_stage(FusedArgs A)
                 r1 = nb_max(r1, nc / b2);
                 double den = nj * sqrt(b2);
                 r2 = nb_max(r2, den > 0 ? nc / den : 0.0);
+                r3 = nb_max(r3, nc);
+                r4 = nb_max(r4, den);
             }
             if constexpr (!FINAL) {
                 if constexpr (stage >= 0) {
@@ -764,11 +766,15 @@ k_stage(FusedArgs A)
         jmax = wave_max(jmax);
         r1 = wave_max(r1);
         r2 = wave_max(r2);
+        r3 = wave_max(r3);
+        r4 = wave_max(r4);
         if (lane == 0) {
             red[1][w] = e_sum;
             red[2][w] = jmax;
             red[3][w] = r1;
             red[4][w] = r2;
+            red[5][w] = r3;
+            red[6][w] = r4;
         }
     }
     __syncthreads();
@@ -779,17 +785,13 @@ k_stage(FusedArgs A)
             A.part_delta[blockIdx.x] = m;
         }
         if (A.diag) {
-            double s = 0, a1 = 0, a2 = 0, a3 = 0;
+            double s = 0, a[5] = {0, 0, 0, 0, 0};
             for (int q = 0; q < NT / 64; ++q) {
                 s += red[1][q];
-                a1 = nb_max(a1, red[2][q]);
-                a2 = nb_max(a2, red[3][q]);
-                a3 = nb_max(a3, red[4][q]);
+                for (int m = 0; m < 5; ++m) a[m] = nb_max(a[m], red[m + 2][q]);
             }
             A.part_diag[blockIdx.x] = s;
-            A.part_diag[A.nblocks + blockIdx.x] = a1;
-            A.part_diag[2 * A.nblocks + blockIdx.x] = a2;
-            A.part_diag[3 * A.nblocks + blockIdx.x] = a3;
+            for (int m = 0; m < 5; ++m) A.part_diag[(m + 1) * A.nblocks + blockIdx.x] = a[m];
         }
     }
 }
@@ -799,12 +801,12 @@ k_stage(FusedArgs A)
 // dt update of src/main.cu:321-383 and logs the attempt.
 __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
 {
-    __shared__ double red[5][16];
+    __shared__ double red[7][16];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const int nb = A.nblocks;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double m = 0, e = 0, jm = 0, r1 = 0, r2 = 0;
+    double m = 0, e = 0, jm = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) {
         m = nb_max(m, A.part_delta[i]);
         if (A.diag) {
@@ -812,6 +814,8 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
             jm = nb_max(jm, A.part_diag[nb + i]);
             r1 = nb_max(r1, A.part_diag[2 * nb + i]);
             r2 = nb_max(r2, A.part_diag[3 * nb + i]);
+            r3 = nb_max(r3, A.part_diag[4 * nb + i]);
+            r4 = nb_max(r4, A.part_diag[5 * nb + i]);
         }
     }
     m = wave_max(m);
@@ -819,23 +823,31 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
     jm = wave_max(jm);
     r1 = wave_max(r1);
     r2 = wave_max(r2);
+    r3 = wave_max(r3);
+    r4 = wave_max(r4);
     if (lane == 0) {
         red[0][w] = m;
         red[1][w] = e;
         red[2][w] = jm;
         red[3][w] = r1;
         red[4][w] = r2;
+        red[5][w] = r3;
+        red[6][w] = r4;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    double md = 0, es = 0, jmx = 0, rr1 = 0, rr2 = 0;
+    double md = 0, es = 0, jmx = 0, rr1 = 0, rr2 = 0, rr3 = 0, rr4 = 0;
     for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
         md = nb_max(md, red[0][q]);
         es += red[1][q];
         jmx = nb_max(jmx, red[2][q]);
         rr1 = nb_max(rr1, red[3][q]);
         rr2 = nb_max(rr2, red[4][q]);
+        rr3 = nb_max(rr3, red[5][q]);
+        rr4 = nb_max(rr4, red[6][q]);
     }
+    // relaxation criterion: max|JxB| / max(|J||B|) (0 when J vanishes)
+    const double crit = rr4 > 0 ? rr3 / rr4 : 0.0;
     StepRecord rec;
     rec.t = ctl->t;
     rec.dt_used = ctl->dt;
@@ -845,16 +857,18 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
         ctl->energy = es * A.cell;
         ctl->jmax = jmx;
         ctl->jxb_b2 = rr1;
-        ctl->ratio = rr2;
+        ctl->ratio = crit;
+        ctl->ratio_pt = rr2;
         ctl->diag_valid = 1;
     }
-    rec.ratio = A.diag ? rr2 : -1;
+    rec.ratio = A.diag ? crit : -1;
+    rec.ratio_pt = A.diag ? rr2 : -1;
     rec.energy = A.diag ? es * A.cell : -1;
     rec.jxb_b2 = A.diag ? rr1 : -1;
     rec.jmax = A.diag ? jmx : -1;
     // the state at the start of this attempt already meets the criterion:
     // stop there and drop the attempt
-    if (A.diag && ctl->stop_ratio > 0 && rr2 <= ctl->stop_ratio) {
+    if (A.diag && ctl->stop_ratio > 0 && crit <= ctl->stop_ratio) {
         ctl->converged = 1;
         ctl->done = 1;
         ctl->stop_reason = 3;

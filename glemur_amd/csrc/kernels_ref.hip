@@ -384,14 +384,15 @@ __global__ void __launch_bounds__(1024) k_max_final(const double *partials, int 
 // ------------------------------------------------------------ diagnostics
 // Per point: energy density |B|^2 det / 2^(#boundary faces)
 // (src/maths.cu:647-680), |J|, |JxB|/|B|^2 (src/maths.cu:168-200) and
-// |JxB|/(|J||B|) (0/0 := 0); grid-stride, per-block partials.
+// |JxB|/(|J||B|) (0/0 := 0), and the maxima of |JxB| and |J||B| (their
+// quotient is the relaxation criterion); grid-stride, per-block partials.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_diag(Geo g, const R *B, const R *J, const double *det, double *part,
                                                  int nblk)
 {
-    __shared__ double red[4][kBlock / 64];
+    __shared__ double red[6][kBlock / 64];
     long long N = (long long)g.nx * g.ny * g.nz;
-    double e = 0, jm = 0, r1 = 0, r2 = 0;
+    double e = 0, jm = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < N; t += (long long)gridDim.x * blockDim.x) {
         int i = (int)(t % g.nx), j = (int)((t / g.nx) % g.ny), k = (int)(t / ((long long)g.nx * g.ny));
         long long a = gidx(g, i + 1, j + 1, k + 1);
@@ -412,31 +413,33 @@ __global__ void __launch_bounds__(kBlock) k_diag(Geo g, const R *B, const R *J, 
         r1 = nb_max(r1, nc / b2);
         double den = nj * nb;
         r2 = nb_max(r2, den > 0 ? nc / den : 0.0);
+        r3 = nb_max(r3, nc);
+        r4 = nb_max(r4, den);
     }
     e = wave_sum(e);
     jm = wave_max(jm);
     r1 = wave_max(r1);
     r2 = wave_max(r2);
+    r3 = wave_max(r3);
+    r4 = wave_max(r4);
     int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[0][w] = e;
         red[1][w] = jm;
         red[2][w] = r1;
         red[3][w] = r2;
+        red[4][w] = r3;
+        red[5][w] = r4;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s = 0, a1 = 0, a2 = 0, a3 = 0;
+        double s = 0, a[5] = {0, 0, 0, 0, 0};
         for (int q = 0; q < kBlock / 64; ++q) {
             s += red[0][q];
-            a1 = nb_max(a1, red[1][q]);
-            a2 = nb_max(a2, red[2][q]);
-            a3 = nb_max(a3, red[3][q]);
+            for (int m = 0; m < 5; ++m) a[m] = nb_max(a[m], red[m + 1][q]);
         }
         part[blockIdx.x] = s;
-        part[nblk + blockIdx.x] = a1;
-        part[2 * nblk + blockIdx.x] = a2;
-        part[3 * nblk + blockIdx.x] = a3;
+        for (int m = 0; m < 5; ++m) part[(m + 1) * nblk + blockIdx.x] = a[m];
     }
 }
 
@@ -444,7 +447,7 @@ __global__ void __launch_bounds__(1024) k_diag_final(Geo g, const double *part, 
 {
     // one wave per quantity, fixed order
     int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (q >= 4) return;
+    if (q >= 6) return;
     double v = 0;
     for (int i = lane; i < nblk; i += 64) v = q == 0 ? v + part[i] : nb_max(v, part[q * nblk + i]);
     v = q == 0 ? wave_sum(v) : wave_max(v);
@@ -581,7 +584,7 @@ void launch_diag(hipStream_t s, const Geo &g, const R *B, const R *J, const doub
     int nb = blocks_for(ninner(g));
     if (nb > max_blocks) nb = max_blocks;
     hipLaunchKernelGGL(k_diag<R>, dim3(nb), dim3(kBlock), 0, s, g, B, J, det, scratch, nb);
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, g, scratch, nb, out);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(6 * 64), 0, s, g, scratch, nb, out);  // one wave per quantity
 }
 template void launch_diag<double>(hipStream_t, const Geo &, const double *, const double *, const double *, double *, int, double *);
 template void launch_diag<float>(hipStream_t, const Geo &, const float *, const float *, const double *, double *, int, double *);

@@ -173,7 +173,7 @@ int ensure_partials(gm_ctx *c)
     dfree(c->part_delta);
     dfree(c->part_diag);
     int r;
-    if ((r = dalloc(&c->part_delta, (size_t)c->nblocks)) || (r = dalloc(&c->part_diag, (size_t)4 * c->nblocks))) return r;
+    if ((r = dalloc(&c->part_delta, (size_t)c->nblocks)) || (r = dalloc(&c->part_diag, (size_t)6 * c->nblocks))) return r;
     c->part_cap = c->nblocks;
     return GM_OK;
 }
@@ -367,7 +367,7 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     c->ctlh->n_cycle = 1LL << 62;
     c->ctlh->max_attempt = 1LL << 62;
     if ((r = dalloc(&c->log, (size_t)c->log_cap))) return bail(r);
-    if ((r = dalloc(&c->red_out, 8)) || (r = dalloc(&c->diag_scratch, (size_t)4 * c->diag_blocks)) ||
+    if ((r = dalloc(&c->red_out, 8)) || (r = dalloc(&c->diag_scratch, (size_t)6 * c->diag_blocks)) ||
         (r = dalloc(&c->nanflag, 1)))
         return bail(r);
     if (hipHostMalloc((void **)&c->red_host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
@@ -638,13 +638,14 @@ extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
         gmd::launch_diag<double>(c->stream, c->geo, c->B64, c->J64, c->det, c->diag_scratch, c->diag_blocks, c->red_out);
     else
         gmd::launch_diag<float>(c->stream, c->geo, c->B32, c->J32, c->det, c->diag_scratch, c->diag_blocks, c->red_out);
-    HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 4 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     memset(red, 0, sizeof *red);
     red->B2 = c->red_host[0];
     red->JMax = c->red_host[1];
     red->JxB_B2Max = c->red_host[2];
     red->JxB_JBMax = c->red_host[3];
+    red->JxBMax_JBMax = c->red_host[5] > 0 ? c->red_host[4] / c->red_host[5] : 0.0;
     red->B2res = gm_init_residual(&c->p);
     return GM_OK;
 }
@@ -752,6 +753,7 @@ extern "C" int gm_run(gm_ctx *c, const gm_run_opts *o, gm_run_stats *st)
         st->t = h.t;
         st->dt = h.dt;
         st->ratio = h.diag_valid ? h.ratio : -1;
+        st->ratio_pt = h.diag_valid ? h.ratio_pt : -1;
         st->energy = h.diag_valid ? h.energy : -1;
         st->converged = h.converged;
         st->stopped = h.done && !h.converged;
@@ -774,7 +776,7 @@ extern "C" int gm_step_log(gm_ctx *c, gm_step_record *out, int n)
     for (long long i = 0; i < cnt; ++i) {
         long long idx = (total - cnt + i) % c->log_cap;
         const gmd::StepRecord &s = all[idx];
-        out[i].t = s.t; out[i].dt_used = s.dt_used; out[i].maxDelta = s.maxDelta; out[i].ratio = s.ratio;
+        out[i].t = s.t; out[i].dt_used = s.dt_used; out[i].maxDelta = s.maxDelta; out[i].ratio = s.ratio; out[i].ratio_pt = s.ratio_pt;
         out[i].energy = s.energy; out[i].jxb_b2 = s.jxb_b2; out[i].jmax = s.jmax;
         out[i].accepted = s.accepted; out[i].nan_seen = s.nan_seen;
     }

@@ -89,12 +89,15 @@ typedef struct gm_params {
     bool   redErrB_1ez, redErrXb_XbAn, redConvex, redWedgeMin, redU2, redUMax;
 } gm_params;
 
-/* Reduction results (src/global.h:111-124) plus the relaxation criterion
- * max|JxB|/(|J||B|) (JxB_JBMax), which the reference does not compute. */
+/* Reduction results (src/global.h:111-124) plus two measures of
+ * force-freeness the reference does not compute: JxB_JBMax, the pointwise
+ * max |JxB|/(|J||B|) (0/0 := 0), and JxBMax_JBMax = max|JxB| / max(|J||B|),
+ * the relaxation criterion (the pointwise form stays near 1 wherever J is
+ * at rounding level, e.g. in a uniform background field). */
 typedef struct gm_reduction {
     double B2, B2res, BMax, JMax, JxB_B2Max, epsilonStar, errB_1ez, errXb_XbAn;
     double convex, wedgeMin, U2, UMax;
-    double JxB_JBMax;
+    double JxB_JBMax, JxBMax_JBMax;
 } gm_reduction;
 
 /* What one attempted step did (src/main.cu:309-383). */
@@ -194,7 +197,7 @@ int gm_step(gm_ctx *ctx, gm_step_info *info);
 typedef struct gm_run_opts {
     int64_t max_attempts;   /* upper bound on attempted steps               */
     int     n_cycle;        /* stop after this many accepted steps (<=0: params.nCycle) */
-    double  stop_ratio;     /* stop when max|JxB|/(|J||B|) <= stop_ratio (<=0: off) */
+    double  stop_ratio;     /* stop when max|JxB| / max(|J||B|) <= stop_ratio (<=0: off) */
     int     batch;          /* steps enqueued between host checks (>=1)     */
     int     use_graph;      /* capture one batch in a hipGraph and replay it */
 } gm_run_opts;
@@ -202,7 +205,8 @@ typedef struct gm_run_opts {
 typedef struct gm_run_stats {
     int64_t n_attempt, n_accept;
     double  t, dt;
-    double  ratio;          /* last evaluated max|JxB|/(|J||B|)              */
+    double  ratio;          /* last evaluated max|JxB| / max(|J||B|)         */
+    double  ratio_pt;       /* last evaluated pointwise max |JxB|/(|J||B|)   */
     double  energy;         /* last evaluated B2                             */
     int     converged, stopped, nan_seen;
     double  seconds;        /* wall time of the run, host clock              */
@@ -216,7 +220,7 @@ int gm_run(gm_ctx *ctx, const gm_run_opts *opts, gm_run_stats *stats);
 
 /* Per-step record kept by the device controller (attempt order). */
 typedef struct gm_step_record {
-    double t, dt_used, maxDelta, ratio, energy, jxb_b2, jmax;
+    double t, dt_used, maxDelta, ratio, ratio_pt, energy, jxb_b2, jmax;
     int32_t accepted, nan_seen;
 } gm_step_record;
 /* Copy the last n records (n <= capacity) into out; returns the count. */
@@ -250,7 +254,8 @@ int gm_dump_state(const char *dir, const gm_params *p, double t, double dt, int 
 int gm_read_state(const char *dir, const gm_params *p, double *xb, double *B0,
                   double *t, double *dt, int precision);
 /* writeTs (src/io.cu:542-678); header when t == 0 (echoed to stdout when
- * echo != 0).  red may be NULL for the header. */
+ * echo != 0).  red may be NULL for the header.  jxb_jb_col != 0 appends the
+ * columns JxB_JBMax and JxBMax_JBMax after the reference's. */
 int gm_write_ts(const char *dir, const gm_params *p, const gm_reduction *red,
                 double t, double dt, int it, double maxDelta, int echo, int jxb_jb_col);
 /* writeCudaInfo (src/io.cu:10-82): device properties into <dir>/cuda.info. */

@@ -81,6 +81,7 @@ _SIGS = {
     "or_jmax": (C.c_double, [_PP, _D]),
     "or_jxb_b2_max": (C.c_double, [_PP, _D, _D]),
     "or_jxb_jb_max": (C.c_double, [_PP, _D, _D]),
+    "or_jxbmax_jbmax": (C.c_double, [_PP, _D, _D]),
     "or_state_create": (C.c_int, [_PP, _D, _D, C.POINTER(State)]),
     "or_state_free": (None, [C.POINTER(State)]),
     "or_attempt": (C.c_double, [_PP, C.POINTER(State), C.POINTER(C.c_int)]),
@@ -181,6 +182,9 @@ class Sim:
 
     def jxb_jb_max(self):
         return _lib.or_jxb_jb_max(C.byref(self.p), self.s.B, self.s.J)
+
+    def jxbmax_jbmax(self):
+        return _lib.or_jxbmax_jbmax(C.byref(self.p), self.s.B, self.s.J)
 
 
 def set_threads(n):

@@ -835,6 +835,28 @@ double or_jxb_jb_max(const or_params *p, const double *B, const double *J)
     return mx;
 }
 
+/* max |JxB| / max (|J||B|).  No reference counterpart: the relaxation
+ * criterion of the workload. */
+double or_jxbmax_jbmax(const or_params *p, const double *B, const double *J)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double mc = 0, md = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double *b = B + 3 * pid(p, i + 1, j + 1, k + 1);
+                const double *jj = J + 3 * iid(p, i, j, k);
+                double c[3];
+                cross3(jj, b, c);
+                double nc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                double den = sqrt(jj[0] * jj[0] + jj[1] * jj[1] + jj[2] * jj[2]) *
+                             sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+                if (nc > mc) mc = nc;
+                if (den > md) md = den;
+            }
+    return md > 0 ? mc / md : 0;
+}
+
 int or_state_create(const or_params *p, const double *B0, const double *xb, or_state *s)
 {
     long P3 = 3 * npad(p), N3 = 3 * ninner(p);

@@ -119,6 +119,9 @@ double or_jmax(const or_params *p, const double *J);
 double or_jxb_b2_max(const or_params *p, const double *B, const double *J);
 /* max |JxB|/(|J||B|) with 0/0 := 0 (no reference counterpart). */
 double or_jxb_jb_max(const or_params *p, const double *B, const double *J);
+/* max |JxB| / max (|J||B|) (0 when J vanishes): the relaxation criterion
+ * (no reference counterpart). */
+double or_jxbmax_jbmax(const or_params *p, const double *B, const double *J);
 
 /* Whole-step driver state (src/main.cu:259-384). */
 typedef struct or_state {

@@ -175,6 +175,7 @@ def test_reductions_match_oracle(g):
     assert r.JMax == sim.jmax()
     assert r.JxB_B2Max == sim.jxb_b2_max()
     assert r.JxB_JBMax == sim.jxb_jb_max()
+    assert r.JxBMax_JBMax == pytest.approx(sim.jxbmax_jbmax(), rel=1e-15)
 
 
 @pytest.mark.parametrize("peri", ["xPeri", "yPeri", "zPeri"])

@@ -132,20 +132,21 @@ def test_read_state_missing_file_is_an_error(tmp_path):
 
 def test_time_series_columns(tmp_path):
     """Header written when t == 0, one row per call after; columns in the
-    reference order, plus the JxB_JBMax column this library adds last."""
+    reference order, plus the two force-freeness columns this library adds
+    last."""
     p = small_params()
     p.redJMax = p.redJxB_B2Max = p.redB2 = True
     p.redEpsilonStar = p.redConvex = False
     os.makedirs(tmp_path / "data")
     g.write_ts(tmp_path, p, None, 0.0, 0.0, 0, 0.0)
     red = g.Reduction()
-    red.JMax, red.JxB_B2Max, red.B2, red.JxB_JBMax = 2.5, 0.125, 384.0, 0.0625
+    red.JMax, red.JxB_B2Max, red.B2, red.JxB_JBMax, red.JxBMax_JBMax = 2.5, 0.125, 384.0, 0.0625, 1.5e-3
     g.write_ts(tmp_path, p, red, 0.5, 1e-3, 7, 3.25e-6)
     lines = open(tmp_path / "data" / "time_series.dat").read().splitlines()
-    assert lines[0] == "#%7s%13s%13s%13s%13s%13s%13s%13s" % ("it", "t", "dt", "maxDelta", "JMax", "JxB_B2Max", "B2",
-                                                              "JxB_JBMax")
-    assert lines[1] == "%8i %12.5e %12.5e %12.5e %12.5e %12.5e %12.5e %12.5e " % (7, 0.5, 1e-3, 3.25e-6, 2.5, 0.125,
-                                                                                   384.0, 0.0625)
+    assert lines[0] == "#%7s%13s%13s%13s%13s%13s%13s%13s%13s" % ("it", "t", "dt", "maxDelta", "JMax", "JxB_B2Max",
+                                                                  "B2", "JxB_JBMax", "JxBMax_JBMax")
+    assert lines[1] == "%8i %12.5e %12.5e %12.5e %12.5e %12.5e %12.5e %12.5e %12.5e " % (
+        7, 0.5, 1e-3, 3.25e-6, 2.5, 0.125, 384.0, 0.0625, 1.5e-3)
     # without the extra column the row is the reference's
     g.write_ts(tmp_path, p, None, 0.0, 0.0, 0, 0.0, jxb_jb_col=False)
     g.write_ts(tmp_path, p, red, 0.5, 1e-3, 7, 3.25e-6, jxb_jb_col=False)

@@ -501,6 +501,13 @@ def test_reductions_brute_force():
         r2 = np.where(nj * np.sqrt(b2) > 0, nc / (nj * np.sqrt(b2)), 0.0)
     assert o.lib.or_jxb_jb_max(ref(p), o.dp(Bf), o.dp(J)) == pytest.approx(np.nanmax(r2), rel=1e-15)
     assert np.nanmax(r2) <= 1.0 + 1e-15
+    want = np.nanmax(nc) / np.nanmax(nj * np.sqrt(b2))
+    assert o.lib.or_jxbmax_jbmax(ref(p), o.dp(Bf), o.dp(J)) == pytest.approx(want, rel=1e-15)
+    # force-free: J parallel to B everywhere gives 0; J = 0 gives 0 (no 0/0)
+    Jff = (Bi * 0.7).reshape(-1).copy()
+    assert o.lib.or_jxbmax_jbmax(ref(p), o.dp(Bf), o.dp(Jff)) < 1e-15
+    Jz = np.zeros_like(J)
+    assert o.lib.or_jxbmax_jbmax(ref(p), o.dp(Bf), o.dp(Jz)) == 0.0
 
 
 @pytest.mark.parametrize("zupdate", [True, False])
