@@ -10,12 +10,19 @@ timed steps (FP32 stages with compensated positions; the FP64 path is
 reported beside it).
 
 roofline: algorithmic bytes of one stage launch (80 B per grid point in
-FP32+compensation, 132 B in FP64: x0, B0 and the earlier slopes read once,
-the new slope written once) over the stage kernel's mean duration, timed
-with HIP events on the library's stream, against 8 TB/s.
+FP32+compensation, 132 B in FP64, averaged over the six stages: x0, B0 and
+the earlier slopes read once, the new slope written once) over the stage
+kernel's mean duration, timed with HIP events on the library's stream,
+against 8 TB/s.  traffic: HBM bytes per stage launch from the rocprofv3
+FETCH_SIZE/WRITE_SIZE summary in profiles/ (calibrated against a 1 GiB
+copy), reported when this run uses the tile it was measured with.
 
 cpu_baseline: the FP64 oracle (a serial/OpenMP C restatement of the
 reference step) on a bounded sample of the same workload on this host.
+
+relaxation: attempted steps, time and force-freeness reached by the F32K
+path in a bounded run from the initial state towards
+max|JxB| / max(|J||B|) <= 1e-3.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -29,45 +36,79 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_braid129x129x385_hbm_traffic.md")
+
+
+def measured_traffic():
+    """{prec: (tile tuple, bytes per launch)} from the committed counter summary."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return {}
+    tiles, tr = {}, {}
+    for l in lines:
+        f = l.split()
+        if len(f) == 5 and f[0].startswith("tile_"):
+            tiles[f[0][5:]] = tuple(int(x) for x in f[1:])
+        elif len(f) == 2 and f[0].startswith("traffic_bytes_per_launch_"):
+            tr[f[0].rsplit("_", 1)[1]] = float(f[1])
+    return {k: (tiles.get(k), v) for k, v in tr.items()}
+
+
+def traffic_for(traffic, prec, tile):
+    t = traffic.get(prec)
+    return t[1] if t and t[0] == tuple(tile) else None
 
 
 def measure(g, p, B0, xb, precision, steps, warmup, dt):
     ctx = g.Context(p, precision)
     ctx.upload(B0, xb, 0.0, dt)
     # warm-up attempts (not timed)
-    ctx.run(max_attempts=warmup, batch=max(1, warmup))
-    # timed region: `steps` attempts enqueued back to back, one host sync
+    ctx.run(max_attempts=warmup, n_cycle=10 ** 9, batch=max(1, warmup))
+    # per-launch HIP-event timing of the stage kernels over `steps` attempts
     ctx.timing(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    st = ctx.run(max_attempts=steps, batch=steps)
-    t1 = time.perf_counter()
+    ctx.run(max_attempts=steps, n_cycle=10 ** 9, batch=steps)
     ms, cnt = ctx.timing_read()
     ctx.timing(False)
-    # untimed rerun without per-launch events for the throughput figure
+    # timed region (no per-launch events): `steps` attempts enqueued back to
+    # back from the same state, one host sync at each end
     ctx.upload(B0, xb, 0.0, dt)
-    ctx.run(max_attempts=warmup, batch=max(1, warmup))
+    ctx.run(max_attempts=warmup, n_cycle=10 ** 9, batch=max(1, warmup))
     ctx.sync()
     t2 = time.perf_counter()
-    st2 = ctx.run(max_attempts=steps, batch=steps)
+    st2 = ctx.run(max_attempts=steps, n_cycle=10 ** 9, batch=steps)
     t3 = time.perf_counter()
     tile = ctx.tile()
     ctx.close()
-    n_att = st2.n_attempt - warmup
+    per_stage = [ms[n] / max(1, cnt[n]) for n in range(6)]
     return {
         "seconds": t3 - t2,
-        "attempts": int(n_att),
+        "attempts": int(st2.n_attempt - warmup),
         "accepted": int(st2.n_accept),
-        "ms_stage": (ms[0] + ms[1]) / max(1, cnt[0] + cnt[1]),
-        "ms_stage_inner": ms[0] / max(1, cnt[0]),
-        "ms_stage_final": ms[1] / max(1, cnt[1]),
-        "ms_controller": ms[2] / max(1, cnt[2]),
-        "launches": int(cnt[0] + cnt[1]),
-        "timed_seconds_with_events": t1 - t0,
-        "tile": tile,
-        "ratio": st2.ratio,
-        "energy": st2.energy,
+        "ms_stage": sum(ms[:6]) / max(1, sum(cnt[:6])),
+        "ms_per_stage": per_stage,
+        "ms_controller": ms[6] / max(1, cnt[6]),
+        "launches": int(sum(cnt[:6])),
+        "tile": tuple(tile),
     }
+
+
+def relaxation_probe(g, p, B0, xb, seconds):
+    ctx = g.Context(p, g.PREC_F32K)
+    ctx.upload(B0, xb, 0.0, p.dt0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    st = None
+    while time.perf_counter() - t0 < seconds:
+        st = ctx.run(max_attempts=500, n_cycle=10 ** 9, stop_ratio=1e-3, batch=100)
+        if st.converged or st.dt <= p.dtMin:
+            break
+    sec = time.perf_counter() - t0
+    ctx.close()
+    return {"criterion": "max|JxB|/max(|J||B|) <= 1e-3", "precision": "F32K", "seconds": sec,
+            "attempts": int(st.n_attempt), "accepted": int(st.n_accept), "t": st.t, "dt": st.dt,
+            "converged": bool(st.converged), "JxBMax_JBMax": st.ratio, "JxB_JBMax_pointwise": st.ratio_pt,
+            "energy": st.energy, "time_to_relax_s": sec if st.converged else None}
 
 
 def cpu_baseline(p, B0, xb, budget_s=20.0):
@@ -90,7 +131,7 @@ def cpu_baseline(p, B0, xb, budget_s=20.0):
         "cores": threads,
         "kind": "port",
         "sample": f"{n} attempted FP64 steps of the 129x129x385 braid from its initial state "
-                  f"(oracle, {threads} OpenMP threads, {dt:.1f} s)",
+                  f"(oracle, gcc -O2 -fopenmp, {threads} OpenMP threads, {dt:.1f} s)",
     }
 
 
@@ -101,6 +142,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--relax-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -126,8 +168,12 @@ def main():
 
     f32 = measure(g, p, B0, xb, g.PREC_F32K, args.steps, args.warmup, dt0)
     value = n_pts * 6 * f32["attempts"] / f32["seconds"]
-    bytes_launch = g.algorithmic_bytes(g.PREC_F32K) * n_pts
+    bpp = g.algorithmic_bytes(g.PREC_F32K)
+    bytes_launch = bpp * n_pts
     achieved = bytes_launch / (f32["ms_stage"] * 1e-3) / 1e9
+    stage_bytes = [(48 + 12 * n) * n_pts for n in range(5)] + [120 * n_pts]
+    traffic = measured_traffic()
+    traffic_f32 = traffic_for(traffic, "f32k", f32["tile"])
 
     out = {
         "metric": "grid-pt RK-stages/s",
@@ -140,19 +186,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 stages, fp32 hi+lo positions, fp64 det/error norm/energy/dt",
+        "dtype": "f32 (stages; positions f32 hi+lo; f64 det, error norm, energy, dt)",
         "data": "synthetic (closed-form braid field, no checkpoint)",
         "config": {"workload": "braid 129x129x385 (6 blobs, line-tied, friction 1, tol 1e-5)",
                    "grid": [p.nx, p.ny, p.nz], "inner_points": n_pts,
-                   "tile_tx_ty_cz_blocks": list(f32["tile"]), "parallelism": "single GPU"},
+                   "tile_tx_ty_cz_workgroups": list(f32["tile"]), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_fused_f32k (fused Cash-Karp stage)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_f32,
+                     "kernel": "k_stage<true,512,n,false> (fused Cash-Karp stage, F32K), mean of stages 0-5",
                      "algorithmic_bytes_per_launch": bytes_launch,
+                     "algorithmic_bytes_per_point": bpp,
                      "mean_launch_ms": f32["ms_stage"],
-                     "mean_launch_ms_stages0_4": f32["ms_stage_inner"],
-                     "mean_launch_ms_final": f32["ms_stage_final"]},
-        "grid_pt_stages_per_s_hbm_pct": 100 * achieved / HBM_PEAK_GBS,
+                     "per_stage_ms": f32["ms_per_stage"],
+                     "per_stage_frac": [b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                        for b, t in zip(stage_bytes, f32["ms_per_stage"])],
+                     "controller_ms": f32["ms_controller"],
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic_f32 else None},
     }
     if copy_gbs is not None:
         out["copy_probe_GBs"] = copy_gbs
@@ -162,7 +211,11 @@ def main():
         a64 = b64 / (f64["ms_stage"] * 1e-3) / 1e9
         out["f64"] = {"value": n_pts * 6 * f64["attempts"] / f64["seconds"],
                       "ms_per_step": 1e3 * f64["seconds"] / f64["attempts"],
-                      "achieved_GBs": a64, "frac": a64 / HBM_PEAK_GBS, "mean_launch_ms": f64["ms_stage"]}
+                      "achieved_GBs": a64, "frac": a64 / HBM_PEAK_GBS, "mean_launch_ms": f64["ms_stage"],
+                      "per_stage_ms": f64["ms_per_stage"],
+                      "traffic": traffic_for(traffic, "f64", f64["tile"])}
+    if args.relax_seconds > 0:
+        out["relaxation"] = relaxation_probe(g, p, B0, xb, args.relax_seconds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, B0, xb)
     print(json.dumps(out))

@@ -22,6 +22,7 @@ if not os.path.exists(LIB_PATH):
 _lib = C.CDLL(LIB_PATH)
 
 GM_OK = 0
+TIMING_KINDS = 7      # gm_timing_read: stages 0..5, controller
 GM_ERR_ARG, GM_ERR_HIP, GM_ERR_IO, GM_ERR_UNSUPPORTED, GM_ERR_NAN, GM_ERR_PARSE = -1, -2, -3, -4, -5, -6
 PREC_F64, PREC_F32K = 0, 1
 FIELD_XB, FIELD_B0, FIELD_B, FIELD_DETJAC, FIELD_J, FIELD_GRADP, FIELD_KK, FIELD_XB_NEW, FIELD_XB_LO = range(9)
@@ -324,8 +325,9 @@ class Context:
         _check(_lib.gm_timing_enable(self._h, int(on)))
 
     def timing_read(self):
-        ms = (C.c_double * 3)()
-        cnt = (C.c_int64 * 3)()
+        """(ms, count) summed per kind: stages 0..5, then the controller."""
+        ms = (C.c_double * TIMING_KINDS)()
+        cnt = (C.c_int64 * TIMING_KINDS)()
         _check(_lib.gm_timing_read(self._h, ms, cnt))
         return list(ms), list(cnt)
 

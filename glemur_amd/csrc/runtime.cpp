@@ -62,8 +62,8 @@ struct gm_ctx {
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_kind;
     int ev_used = 0;
-    double tms[3] = {0, 0, 0};
-    long long tcount[3] = {0, 0, 0};
+    double tms[GM_TIMING_KINDS] = {};        // per stage 0..5, controller
+    long long tcount[GM_TIMING_KINDS] = {};
     // graph of one batch
     hipGraphExec_t gexec = nullptr;
     int gbatch = 0;
@@ -241,7 +241,7 @@ void enqueue_step(gm_ctx *c, int diag, int check_done)
     int f32k = c->prec == GM_PREC_F32K;
     for (int s = 0; s < 6; ++s) {
         gmd::FusedArgs A = fused_args(c, s, 0, s == 0 ? diag : 0, check_done);
-        ev_begin(c, s < 5 ? 0 : 1);
+        ev_begin(c, s);
         gmd::launch_fused(c->stream, A, f32k);
         ev_end(c);
     }
@@ -257,7 +257,7 @@ void enqueue_step(gm_ctx *c, int diag, int check_done)
     C.cell = c->p.dx * c->p.dy * c->p.dz;
     C.log = c->log;
     C.log_cap = c->log_cap;
-    ev_begin(c, 2);
+    ev_begin(c, 6);
     gmd::launch_controller(c->stream, C);
     ev_end(c);
 }
@@ -790,15 +790,15 @@ extern "C" int gm_timing_enable(gm_ctx *c, int on)
     HIPCHK(hipStreamSynchronize(c->stream));
     ev_collect(c);
     c->timing = on != 0;
-    for (int i = 0; i < 3; ++i) { c->tms[i] = 0; c->tcount[i] = 0; }
+    for (int i = 0; i < GM_TIMING_KINDS; ++i) { c->tms[i] = 0; c->tcount[i] = 0; }
     return GM_OK;
 }
-extern "C" int gm_timing_read(gm_ctx *c, double ms[3], int64_t count[3])
+extern "C" int gm_timing_read(gm_ctx *c, double ms[GM_TIMING_KINDS], int64_t count[GM_TIMING_KINDS])
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
     ev_collect(c);
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < GM_TIMING_KINDS; ++i) {
         if (ms) ms[i] = c->tms[i];
         if (count) count[i] = c->tcount[i];
     }

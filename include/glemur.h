@@ -227,11 +227,13 @@ typedef struct gm_step_record {
 int gm_step_log(gm_ctx *ctx, gm_step_record *out, int n);
 
 /* ---- kernel timing (HIP events on the context stream) ---------------- */
-/* Enable/disable event timing of the fused stage kernels; gm_timing_read
- * returns the summed milliseconds and launch count per stage kind
- * (index 0: stages 0-4, 1: final stage, 2: controller). */
+/* Enable/disable event timing of the fused stage kernels (resets the
+ * totals); gm_timing_read returns the summed milliseconds and launch count
+ * per kind: index 0..5 the stage kernels, 6 the step controller.  Graph
+ * capture is off while timing. */
+#define GM_TIMING_KINDS 7
 int gm_timing_enable(gm_ctx *ctx, int on);
-int gm_timing_read(gm_ctx *ctx, double ms[3], int64_t count[3]);
+int gm_timing_read(gm_ctx *ctx, double ms[GM_TIMING_KINDS], int64_t count[GM_TIMING_KINDS]);
 /* Bytes per grid point and stage the fused kernel must move (algorithmic). */
 double gm_algorithmic_bytes_per_point_stage(int precision);
 /* Plain device copy of `bytes`, `reps` times: mean kernel ms (the yardstick
