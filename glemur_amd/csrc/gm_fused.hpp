@@ -28,6 +28,8 @@ struct FusedArgs {
     int stage;                        // 0..5, or -1 for fields only
     int TX, TY, CZ, tiles_x, tiles_y, nchunks, nblocks;
     int emit, diag, check_done;
+    int face_table;                   // 1: every ghost face from the precomputed constants
+                                      // (initial boundary distortion); 0: x/y faces direct (F32K)
     // F64 buffers
     double *X64[3];                   // positions: current, candidate, stage-6 (3P each)
     const double *B0_64;
@@ -65,6 +67,11 @@ void set_tableau(const TableauF &t);
 void launch_ghost_setup(hipStream_t s, const FusedArgs &A, int f32k, GhostPt *out);
 long long ghost_points(const Geo &g, long long off[6]);
 void launch_fused(hipStream_t s, const FusedArgs &A, int f32k);
+// FP32 stage kernel with compensated positions (kernels_f32.hip).
+void set_tableau32(const TableauF &t);
+void launch_fused32(hipStream_t s, const FusedArgs &A);
+// Workgroup size the FP32 kernel uses for a rect of `rect` points.
+int threads32(int rect);
 void launch_controller(hipStream_t s, const CtlArgs &A);
 
 }  // namespace gmd

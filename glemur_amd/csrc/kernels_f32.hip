@@ -1,0 +1,573 @@
+// Fused Cash-Karp stage kernel, FP32 with compensated positions (F32K).
+//
+// Same 2.5D march as the FP64 kernel (kernels_fused.hip): a workgroup owns
+// an x-y tile plus a 2-point halo (one thread per column) and a z chunk,
+// and per plane p it forms the stage position of plane p+1, issues the HBM
+// loads of plane p+2, evaluates B at plane p and J, force and slope at
+// plane p-1, with one barrier per plane.  What differs, for FP32:
+//   * a point's position (hi x, y, z + lo x | lo y, z) is one 16-byte and
+//     one 8-byte LDS word, B one 16-byte word: a neighbour costs two LDS
+//     reads instead of six;
+//   * a column's own z-neighbours are read back from the LDS rings (4
+//     position planes, 3 B planes) instead of being queued in registers, so
+//     every stage fits 128 VGPRs (two 512-thread workgroups per CU);
+//   * ghost faces x = const and y = const take their tangential Jacobian
+//     columns from the ghost layer itself (in LDS) and the outer ghost from
+//     x_ghost +- h n, with no table loads inside the march; the table of
+//     precomputed face constants is used for the z faces (first and last
+//     plane only) and for grids with an initial boundary distortion;
+//   * the final stage carries, per plane, sum_m (c_m - b5_m) k_m and
+//     sum_m (c_m - c*_m) k_m over the five stored slopes and adds
+//     c_5 k_5 at the point: x_new = x_5 + sum_m (c_m - b5_m) k_m by a
+//     two-sum into (hi, lo).
+#include "fused_common.hpp"
+#include "gm_device.hpp"
+#include "gm_fused.hpp"
+
+namespace gmd {
+
+__constant__ TableauF c_tab32;
+
+namespace {
+
+constexpr int kNXL = 4;      // position planes in LDS
+constexpr int kNBL = 3;      // B planes in LDS
+
+struct Pt3 {
+    float h[3], s[3];        // high words and small parts (lo + stage increment)
+};
+
+__device__ __forceinline__ float dif(const Pt3 &a, const Pt3 &b, int l)
+{
+    return (a.h[l] - b.h[l]) + (a.s[l] - b.s[l]);
+}
+
+template <int NT>
+struct Lds {
+    float4 *xa;   // [kNXL][NT] hi x, y, z, small x
+    float2 *xb;   // [kNXL][NT] small y, z
+    float4 *bl;   // [kNBL][NT] B x, y, z
+    __device__ __forceinline__ static int xs(int q) { return (unsigned)(q + 4 * kNXL) % (unsigned)kNXL; }
+    __device__ __forceinline__ static int bs(int q) { return (unsigned)(q + 4 * kNBL) % (unsigned)kNBL; }
+    __device__ __forceinline__ void ldx(int q, int pt, Pt3 &o) const
+    {
+        const int i = xs(q) * NT + pt;
+        float4 a = xa[i];
+        float2 b = xb[i];
+        o.h[0] = a.x; o.h[1] = a.y; o.h[2] = a.z;
+        o.s[0] = a.w; o.s[1] = b.x; o.s[2] = b.y;
+    }
+    __device__ __forceinline__ void stx(int q, int pt, const Pt3 &v) const
+    {
+        const int i = xs(q) * NT + pt;
+        xa[i] = make_float4(v.h[0], v.h[1], v.h[2], v.s[0]);
+        xb[i] = make_float2(v.s[1], v.s[2]);
+    }
+    __device__ __forceinline__ void ldb(int q, int pt, float o[3]) const
+    {
+        float4 a = bl[bs(q) * NT + pt];
+        o[0] = a.x; o[1] = a.y; o[2] = a.z;
+    }
+    __device__ __forceinline__ void stb(int q, int pt, const float v[3]) const
+    {
+        bl[bs(q) * NT + pt] = make_float4(v[0], v[1], v[2], 0.f);
+    }
+};
+
+template <int NT>
+constexpr size_t lds_bytes32()
+{
+    return (size_t)kNXL * NT * (sizeof(float4) + sizeof(float2)) + (size_t)kNBL * NT * sizeof(float4);
+}
+
+// det of an FP32 Jacobian in FP64 by cofactors (products of two floats are
+// exact in FP64, so each cofactor carries a single rounding)
+__device__ __forceinline__ double det32(const float a[3][3])
+{
+    double c0 = fma((double)a[1][1], (double)a[2][2], -((double)a[1][2] * (double)a[2][1]));
+    double c1 = fma((double)a[1][2], (double)a[2][0], -((double)a[1][0] * (double)a[2][2]));
+    double c2 = fma((double)a[1][0], (double)a[2][1], -((double)a[1][1] * (double)a[2][0]));
+    return fma((double)a[0][0], c0, fma((double)a[0][1], c1, (double)a[0][2] * c2));
+}
+
+// FP32 Jacobian -> B = jac.B0 / det with det in FP64 (normal < 0: none held)
+__device__ __forceinline__ double cauchy32(const float jac[3][3], const float b0[3], int normal, float B[3])
+{
+    double det = det32(jac);
+    float rd = 1.0f / (float)det;
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+        B[l] = l == normal ? b0[l] : (jac[l][0] * b0[0] + jac[l][1] * b0[1] + jac[l][2] * b0[2]) * rd;
+    return det;
+}
+
+}  // namespace
+
+// Waves per SIMD each workgroup size is register-allocated for: two
+// workgroups per CU (LDS allows two at 512 threads), i.e. 128 VGPRs at 256
+// threads, 168 at 384; 512-thread workgroups keep the compiler's choice.
+constexpr int waves32(int nt) { return nt <= 256 ? 4 : nt <= 384 ? 3 : 2; }
+
+template <int NT, int STAGE, bool PRESS>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT), 8))) k_stage32(FusedArgs A)
+{
+#pragma clang fp contract(fast)
+    constexpr bool FINAL = STAGE == 5;
+    constexpr int stage = STAGE == 6 ? -1 : STAGE;
+    constexpr int n = STAGE == 6 ? 0 : STAGE;   // slopes entering the stage position
+    constexpr bool DIAG = STAGE == 0;            // diagnostics ride on stage 0 only
+
+    extern __shared__ __align__(16) unsigned char smem[];
+    Lds<NT> L;
+    L.xa = reinterpret_cast<float4 *>(smem);
+    L.xb = reinterpret_cast<float2 *>(L.xa + kNXL * NT);
+    L.bl = reinterpret_cast<float4 *>(L.xb + kNXL * NT);
+    __shared__ double red[7][NT / 64];
+
+    const Geo &g = A.g;
+    Ctl *ctl = A.ctl;
+    if (A.check_done && ctl->done) return;
+    const int cur = ctl->cur;
+    const float dt = (float)ctl->dt;
+    const long long P = g.P, P3 = 3 * P;
+
+    const int Lb = xcd_remap(blockIdx.x, A.nblocks);
+    const int tx = Lb % A.tiles_x, ty = (Lb / A.tiles_x) % A.tiles_y, ch = Lb / (A.tiles_x * A.tiles_y);
+    const int ix0 = tx * A.TX, iy0 = ty * A.TY;
+    const int Ks = ch * A.CZ + 1, Ke = min(ch * A.CZ + A.CZ, g.nz);
+    const int RW = A.TX + 4, RH = A.TY + 4;
+    const int ixe = min(ix0 + A.TX, g.nx), iye = min(iy0 + A.TY, g.ny);
+
+    // ---- this thread's column
+    const int t = threadIdx.x;
+    const int c = t % RW, r = t / RW;
+    const int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
+    const bool col = t < RW * RH && gi >= 0 && gi <= g.nx + 1 && gj >= 0 && gj <= g.ny + 1;
+    const bool gx = gi == 0 || gi == g.nx + 1, gy = gj == 0 || gj == g.ny + 1;
+    const bool inB = col && c >= 1 && c <= RW - 2 && r >= 1 && r <= RH - 2 &&
+                     !((c == 1 || c == RW - 2) && (r == 1 || r == RH - 2)) && !(gx && gy);
+    const bool inJ = col && c >= 2 && c <= RW - 3 && r >= 2 && r <= RH - 3 && gi >= 1 && gi <= ixe && gj >= 1 &&
+                     gj <= iye;
+    const long long abase = (long long)gi + g.sx * (long long)gj;
+    const int ii = gi - 1, jj = gj - 1;
+    const float eps_xy = (g.epsilonProf && inJ) ? (float)(A.epsx[ii] * A.epsy[jj]) : 1.0f;
+    const float *Xh0 = A.Xh[cur], *Xl0 = A.Xl[cur];
+    const float *B0 = A.B0_32;
+    const float *K = A.K32;
+
+    // ---- registers carried between planes
+    float b0q[2][3];      // B0 at planes p, p+1
+    double dq[2];         // det at planes p-1, p
+    struct In {
+        float x[3], lo[3], k[5][3], b0[3];
+    } pf;                 // inputs of one plane, loaded one iteration ahead
+    float fg[3][3], fe[3][3];   // final stage: per plane sum (c - b5) k and sum (c - c*) k
+    (void)fg; (void)fe;
+
+    double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
+    int nan_seen = 0;
+
+    auto plane_ok = [&](int q) { return q >= max(Ks - 2, 0) && q <= min(Ke + 2, g.nz + 1); };
+    auto ghost_pt = [&](int q) { return gx || gy || q == 0 || q == g.nz + 1; };
+
+    auto load_plane = [&](int q, In &in) {
+        const long long a = abase + g.sxy * (long long)q;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            in.x[l] = Xh0[l * P + a];
+            in.lo[l] = Xl0[l * P + a];
+            in.b0[l] = B0[l * P + a];
+        }
+        if (!ghost_pt(q)) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+                if (m < n)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) in.k[m][l] = K[m * P3 + l * P + a];
+        }
+    };
+
+    // ---- march
+    const int p_first = Ks - 4, p_last = Ke + 1;
+    for (int p = p_first; p <= p_last; ++p) {
+        // (1) stage position of plane p+1 from the inputs loaded last iteration
+        Pt3 xn;
+        if (col && plane_ok(p + 1)) {
+            const bool gh = ghost_pt(p + 1);
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                float s = pf.lo[l];
+                if (!gh)
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        if (m < n) s += (float)c_tab32.b[n * 6 + m] * pf.k[m][l];
+                xn.h[l] = pf.x[l];
+                xn.s[l] = s;
+                b0q[1][l] = pf.b0[l];
+                if constexpr (FINAL) {
+                    float a1 = 0, a2 = 0;
+                    if (!gh)
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) {
+                            a1 += (float)(c_tab32.c[m] - c_tab32.b[5 * 6 + m]) * pf.k[m][l];
+                            a2 += (float)c_tab32.ce[m] * pf.k[m][l];
+                        }
+                    fg[2][l] = a1;
+                    fe[2][l] = a2;
+                }
+            }
+            L.stx(p + 1, t, xn);
+        }
+        // (0) loads of plane p+2, in flight while B and J are computed
+        if (col && plane_ok(p + 2)) load_plane(p + 2, pf);
+        __builtin_amdgcn_sched_barrier(0);
+
+        // (2) B at (column, p); ghost edges keep B0 and are never read by J
+        if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1 &&
+            !((gx || gy) && (p == 0 || p == g.nz + 1))) {
+            const bool zg = p == 0 || p == g.nz + 1;
+            float Bv[3];
+            double detv;
+            Pt3 xc, xzp, xzm;
+            if (!zg) {
+                L.ldx(p, t, xc);
+                xzp = xn;                 // own plane p+1, formed above
+                L.ldx(p - 1, t, xzm);
+            }
+            if (!gx && !gy && !zg) {
+                // Jacobian column by column, one neighbour pair at a time
+                float jac[3][3];
+                const float hx = 0.5f * (float)g.dx1, hy = 0.5f * (float)g.dy1, hz = 0.5f * (float)g.dz1;
+                {
+                    Pt3 a, b;
+                    L.ldx(p, t + 1, a);
+                    L.ldx(p, t - 1, b);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) jac[l][0] = dif(a, b, l) * hx;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    Pt3 a, b;
+                    L.ldx(p, t + RW, a);
+                    L.ldx(p, t - RW, b);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) jac[l][1] = dif(a, b, l) * hy;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) jac[l][2] = dif(xzp, xzm, l) * hz;
+                detv = cauchy32(jac, b0q[0], -1, Bv);
+            } else if (!zg && !A.face_table) {
+                // x or y face, line-tied: tangential columns from the ghost
+                // layer, normal column from the inner neighbour and the
+                // outer ghost x_g +- h n (src/boundaries.cu:164-186, 291-313)
+                const int face = gx ? 0 : 1;
+                const int side = gx ? (gi != 0) : (gj != 0);
+                const int st = face == 0 ? 1 : RW;
+                Pt3 xin, ta, tb;
+                L.ldx(p, side ? t - st : t + st, xin);
+                L.ldx(p, t + (face == 0 ? RW : 1), ta);   // + along the in-plane tangent
+                L.ldx(p, t - (face == 0 ? RW : 1), tb);
+                const float h = (float)(face == 0 ? g.dx : g.dy);
+                const float h1n = 0.5f * (float)(face == 0 ? g.dx1 : g.dy1);
+                const float h1t = 0.5f * (float)(face == 0 ? g.dy1 : g.dx1);
+                const float hz = 0.5f * (float)g.dz1;
+                const int tcol = face == 0 ? 1 : 0;
+                float jac[3][3];
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    float dn = side ? dif(xc, xin, l) : dif(xin, xc, l);
+                    if (l == face) dn += h;
+                    jac[l][face] = dn * h1n;
+                    jac[l][tcol] = dif(ta, tb, l) * h1t;
+                    jac[l][2] = dif(xzp, xzm, l) * hz;
+                }
+                detv = cauchy32(jac, b0q[0], face, Bv);
+            } else {
+                // z faces (and every face of a grid with an initial boundary
+                // distortion): precomputed face constants
+                int face, side, u, v;
+                double xin[3];
+                if (zg) {
+                    face = 2;
+                    side = p != 0;
+                    u = ii;
+                    v = jj;
+                    Pt3 pin;
+                    if (side) L.ldx(p - 1, t, pin);
+                    else pin = xn;
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) xin[l] = (double)pin.h[l] + (double)pin.s[l];
+                } else {
+                    face = gx ? 0 : 1;
+                    side = gx ? (gi != 0) : (gj != 0);
+                    u = gx ? jj : ii;
+                    v = p - 1;
+                    const int st = face == 0 ? 1 : RW;
+                    Pt3 pin;
+                    L.ldx(p, side ? t - st : t + st, pin);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) xin[l] = (double)pin.h[l] + (double)pin.s[l];
+                }
+                const GhostPt &G = A.ghost[A.ghost_off[face * 2 + side] + u + (long long)nface(g, face) * v];
+                double Bd[3];
+                ghost_b(g, G, face, side, xin, Bd, detv);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) Bv[l] = (float)Bd[l];
+            }
+            if (A.emit && (gx || gy || zg)) {
+                int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
+                if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
+                    long long a = abase + g.sxy * (long long)p;
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) A.eB32[l * P + a] = Bv[l];
+                    A.edet[a] = detv;
+                }
+            }
+            L.stb(p, t, Bv);
+            dq[1] = detv;
+        }
+
+        __builtin_amdgcn_sched_barrier(0);
+        // (3) J, force and slope at (column, p-1): in-plane data of plane p-1
+        // was published before the previous barrier, B(p) is this thread's own
+        const int pj = p - 1;
+        if (inJ && pj >= Ks && pj <= Ke) {
+            const long long a = abase + g.sxy * (long long)pj;
+            // neighbours one at a time, each reduced to its offset from the
+            // point (r = x_v - x_c, b = B_v - B_c) before the next is read:
+            // the scheduler fence keeps the register footprint at one
+            // neighbour instead of all six
+            Pt3 xc;
+            float Bc[3];
+            L.ldx(pj, t, xc);
+            L.ldb(pj, t, Bc);
+            float rr[6][3], bb[6][3];
+            const int vq[6] = {pj, pj, pj, pj, pj - 1, pj + 1};
+            const int vt[6] = {t - 1, t - RW, t + 1, t + RW, t, t};   // xm, ym, xp, yp, zm, zp
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                Pt3 xv;
+                float bv[3];
+                L.ldx(vq[q], vt[q], xv);
+                L.ldb(vq[q], vt[q], bv);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    rr[q][l] = dif(xv, xc, l);
+                    bb[q][l] = bv[l] - Bc[l];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            float Jv[3], kv[3];
+            stokes_f32(rr, bb, Jv);
+            kv[0] = Jv[1] * Bc[2] - Jv[2] * Bc[1];
+            kv[1] = Jv[2] * Bc[0] - Jv[0] * Bc[2];
+            kv[2] = Jv[0] * Bc[1] - Jv[1] * Bc[0];
+            const int kk = pj - 1;
+            float eps = 1.f;
+            if (g.epsilonProf) eps = eps_xy * (float)A.epsz[kk];
+            float gv[3] = {0, 0, 0};
+            const double detv = dq[0];
+            if constexpr (PRESS) {
+                // gradP from second differences (src/gradP.cu:93-154)
+                float jac[3][3], d2[3][3][3];
+                const float h1[3] = {(float)g.dx1, (float)g.dy1, (float)g.dz1};
+                // rr rows: 0 x-, 1 y-, 2 x+, 3 y+, 4 z-, 5 z+ (offsets from x_c)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    jac[l][0] = (rr[2][l] - rr[0][l]) * 0.5f * h1[0];
+                    jac[l][1] = (rr[3][l] - rr[1][l]) * 0.5f * h1[1];
+                    jac[l][2] = (rr[5][l] - rr[4][l]) * 0.5f * h1[2];
+                    d2[l][0][0] = (rr[2][l] + rr[0][l]) * h1[0] * h1[0];
+                    d2[l][1][1] = (rr[3][l] + rr[1][l]) * h1[1] * h1[1];
+                    d2[l][2][2] = (rr[5][l] + rr[4][l]) * h1[2] * h1[2];
+                }
+                // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b))
+                Pt3 pa, pb, pc, pd;
+                // xy
+                L.ldx(pj, t + 1 + RW, pa); L.ldx(pj, t + 1 - RW, pb);
+                L.ldx(pj, t - 1 + RW, pc); L.ldx(pj, t - 1 - RW, pd);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d2[l][0][1] = d2[l][1][0] = (dif(pa, pb, l) - dif(pc, pd, l)) * h1[0] * h1[1] * 0.25f;
+                // xz
+                L.ldx(pj + 1, t + 1, pa); L.ldx(pj - 1, t + 1, pb);
+                L.ldx(pj + 1, t - 1, pc); L.ldx(pj - 1, t - 1, pd);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d2[l][0][2] = d2[l][2][0] = (dif(pa, pb, l) - dif(pc, pd, l)) * h1[0] * h1[2] * 0.25f;
+                // yz
+                L.ldx(pj + 1, t + RW, pa); L.ldx(pj - 1, t + RW, pb);
+                L.ldx(pj + 1, t - RW, pc); L.ldx(pj - 1, t - RW, pd);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d2[l][1][2] = d2[l][2][1] = (dif(pa, pb, l) - dif(pc, pd, l)) * h1[1] * h1[2] * 0.25f;
+                gradp_core(jac, d2, (float)(1 / detv), (float)g.beta, gv);
+            }
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                kv[l] *= dt * eps;
+                if (PRESS) kv[l] += -dt * gv[l];
+            }
+            if (A.emit) {
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    A.eB32[l * P + a] = Bc[l];
+                    A.eJ32[l * P + a] = Jv[l];
+                    if (PRESS) A.eG32[l * P + a] = gv[l];
+                }
+                A.edet[a] = detv;
+            }
+            if (DIAG && A.diag) {
+                double Bd[3] = {(double)Bc[0], (double)Bc[1], (double)Bc[2]};
+                double Jd[3] = {(double)Jv[0], (double)Jv[1], (double)Jv[2]}, jxb[3];
+                cross3(Jd, Bd, jxb);
+                double b2 = Bd[0] * Bd[0] + Bd[1] * Bd[1] + Bd[2] * Bd[2];
+                double nc = sqrt(jxb[0] * jxb[0] + jxb[1] * jxb[1] + jxb[2] * jxb[2]);
+                double nj = sqrt(Jd[0] * Jd[0] + Jd[1] * Jd[1] + Jd[2] * Jd[2]);
+                double v = b2 * detv;
+                int bound = (ii == 0) + (jj == 0) + (kk == 0) + (ii == g.nx - 1) + (jj == g.ny - 1) + (kk == g.nz - 1);
+                v /= (1 << bound);
+                e_sum += v;
+                jmax = nb_max(jmax, nj);
+                r1 = nb_max(r1, nc / b2);
+                double den = nj * sqrt(b2);
+                r2 = nb_max(r2, den > 0 ? nc / den : 0.0);
+                r3 = nb_max(r3, nc);
+                r4 = nb_max(r4, den);
+            }
+            if constexpr (!FINAL) {
+                if constexpr (stage >= 0) {
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) A.K32[stage * P3 + l * P + a] = kv[l];
+                }
+            } else {
+                // x_new = x_5 + sum_m (c_m - b5_m) k_m (k_5 included), into
+                // (hi, lo) by a two-sum; error |sum_m (c_m - c*_m) k_m| in FP64
+                float *Xhd = A.Xh[cur ^ 1], *Xld = A.Xl[cur ^ 1];
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    float hi = xc.h[l], lo;
+                    if (l < 2 || g.zUpdate) {
+                        float d = xc.s[l] + (fg[0][l] + (float)c_tab32.c[5] * kv[l]);
+                        float s, e;
+                        two_sum(hi, d, s, e);
+                        float hi2 = s + e;
+                        lo = e - (hi2 - s);
+                        hi = hi2;
+                        double dd = fabs((double)fe[0][l] + c_tab32.ce[5] * (double)kv[l]);
+                        if (dd != dd) nan_seen = 1;
+                        dmax = nb_max(dmax, dd);
+                    } else {
+                        // z held: x_5's small part is the old lo word (no slope terms on z)
+                        lo = A.Xl[cur][l * P + a];
+                    }
+                    Xhd[l * P + a] = hi;
+                    Xld[l * P + a] = lo;
+                }
+            }
+        }
+
+        // one barrier per plane
+        __syncthreads();
+
+#pragma unroll
+        for (int l = 0; l < 3; ++l) b0q[0][l] = b0q[1][l];
+        dq[0] = dq[1];
+        if constexpr (FINAL) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                fg[0][l] = fg[1][l]; fg[1][l] = fg[2][l];
+                fe[0][l] = fe[1][l]; fe[1][l] = fe[2][l];
+            }
+        }
+    }
+
+    // ---- block partials (fixed order: deterministic)
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (FINAL) {
+        dmax = wave_max(dmax);
+        if (lane == 0) red[0][w] = dmax;
+        if (nan_seen) atomicOr(&ctl->nan_seen, 1);
+    }
+    if (DIAG && A.diag) {
+        e_sum = wave_sum(e_sum);
+        jmax = wave_max(jmax);
+        r1 = wave_max(r1);
+        r2 = wave_max(r2);
+        r3 = wave_max(r3);
+        r4 = wave_max(r4);
+        if (lane == 0) {
+            red[1][w] = e_sum;
+            red[2][w] = jmax;
+            red[3][w] = r1;
+            red[4][w] = r2;
+            red[5][w] = r3;
+            red[6][w] = r4;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (FINAL) {
+            double m = 0;
+            for (int q = 0; q < NT / 64; ++q) m = nb_max(m, red[0][q]);
+            A.part_delta[blockIdx.x] = m;
+        }
+        if (DIAG && A.diag) {
+            double s = 0, am[5] = {0, 0, 0, 0, 0};
+            for (int q = 0; q < NT / 64; ++q) {
+                s += red[1][q];
+                for (int m = 0; m < 5; ++m) am[m] = nb_max(am[m], red[m + 2][q]);
+            }
+            A.part_diag[blockIdx.x] = s;
+            for (int m = 0; m < 5; ++m) A.part_diag[(m + 1) * A.nblocks + blockIdx.x] = am[m];
+        }
+    }
+}
+
+namespace {
+template <int NT, int STAGE, bool PRESS>
+void launch_one32(hipStream_t s, const FusedArgs &A)
+{
+    static bool attr = false;
+    constexpr size_t lds = lds_bytes32<NT>();
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_stage32<NT, STAGE, PRESS>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_stage32<NT, STAGE, PRESS>), dim3(A.nblocks), dim3(NT), lds, s, A);
+}
+template <int NT, bool PRESS>
+void launch_stage32(hipStream_t s, const FusedArgs &A)
+{
+    switch (A.stage) {
+    case 0: launch_one32<NT, 0, PRESS>(s, A); break;
+    case 1: launch_one32<NT, 1, PRESS>(s, A); break;
+    case 2: launch_one32<NT, 2, PRESS>(s, A); break;
+    case 3: launch_one32<NT, 3, PRESS>(s, A); break;
+    case 4: launch_one32<NT, 4, PRESS>(s, A); break;
+    case 5: launch_one32<NT, 5, PRESS>(s, A); break;
+    default: launch_one32<NT, 6, PRESS>(s, A); break;
+    }
+}
+template <int NT>
+void launch_nt32(hipStream_t s, const FusedArgs &A)
+{
+    if (A.g.pressure) launch_stage32<NT, true>(s, A);
+    else launch_stage32<NT, false>(s, A);
+}
+}  // namespace
+
+int threads32(int rect) { return rect <= 256 ? 256 : rect <= 384 ? 384 : 512; }
+
+void set_tableau32(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &t, sizeof t); }
+
+void launch_fused32(hipStream_t s, const FusedArgs &A)
+{
+    // the smallest workgroup that holds the rect (the caller checked <= 512)
+    switch (threads32((A.TX + 4) * (A.TY + 4))) {
+    case 256: launch_nt32<256>(s, A); break;
+    case 384: launch_nt32<384>(s, A); break;
+    default: launch_nt32<512>(s, A); break;
+    }
+}
+
+}  // namespace gmd

@@ -22,62 +22,19 @@
 // So per point and stage HBM sees x0, B0 and the earlier slopes once and the
 // new slope once; B, J and x_n stay on chip.
 //
-// Two arithmetics: F64 follows the reference's operation order (incremental
-// stage positions of src/rungeKutta.cu:216-227 included) and matches the
-// FP64 oracle bit for bit; F32K keeps positions as FP32 hi + lo words, forms
-// position differences from hi and lo separately and centres the Stokes
-// circulation on the point's own field, with the determinant, error norm,
-// energy and dt in FP64.
+// This file holds the FP64 kernel, which follows the reference's operation
+// order (incremental stage positions of src/rungeKutta.cu:216-227 included)
+// and matches the FP64 oracle bit for bit; the FP32 kernel with compensated
+// positions is in kernels_f32.hip.
 #include "gm_device.hpp"
 #include "gm_fused.hpp"
+#include "fused_common.hpp"
 
 namespace gmd {
 
 __constant__ TableauF c_tab;
 
 namespace {
-
-// Workgroup id -> logical tile so that the blocks one XCD receives
-// (hardware deals ids round-robin over the 8 XCDs) form a contiguous range
-// of tiles and share halos in that XCD's L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nb)
-{
-    int q = nb >> 3, r = nb & 7;
-    int x = bid & 7, w = bid >> 3;
-    int base = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
-    return base + w;
-}
-
-// Two-sum (Knuth): s + e == a + b exactly.
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
-{
-    s = a + b;
-    float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-__device__ __forceinline__ int nface(const Geo &g, int face) { return face == 0 ? g.ny : g.nx; }
-
-// B and det at a line-tied ghost-face point from its first inner neighbour
-// x_in along the normal and the precomputed constants (src/boundaries.cu:
-// 164-186): jac = [normal column from x_in and the outer ghost | tangential
-// columns], B_normal = B0_normal.
-__device__ __forceinline__ void ghost_b(const Geo &g, const GhostPt &G, int face, int side, const double xin[3],
-                                       double B[3], double &det)
-{
-    const double h1 = face == 0 ? g.dx1 : face == 1 ? g.dy1 : g.dz1;
-    const int t0 = face == 0 ? 1 : 0, t1 = face == 2 ? 1 : 2;
-    double jac[3][3], d[3];
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-        d[l] = side ? G.x2[l] - xin[l] : xin[l] - G.x2[l];
-        jac[l][t0] = G.tc0[l];
-        jac[l][t1] = G.tc1[l];
-    }
-    jac_col(jac, face, d, h1);
-    double detJac1 = cauchy_b(jac, G.b0, face, B);
-    det = 1 / detJac1;
-}
 
 // ----------------------------------------------------------- ghost setup
 // Tangential Jacobian columns of a ghost-face point: in-plane neighbours are
@@ -168,107 +125,27 @@ __global__ void k_ghost_setup(FusedArgs A, GhostPt *out)
     out[A.ghost_off[fs] + u + (long long)nu * v] = G;
 }
 
-// Per-precision position type and operations.
-template <bool F32>
-struct PrecT;
-template <>
-struct PrecT<false> {
-    using R = double;
-    struct Pos {
-        double x[3];
-    };
-    __device__ static double diff(const Pos &a, const Pos &b, int l) { return a.x[l] - b.x[l]; }
-    __device__ static double dbl(const Pos &a, int l) { return a.x[l]; }
+struct Pos {
+    double x[3];
 };
-template <>
-struct PrecT<true> {
-    using R = float;
-    struct Pos {
-        float h[3], s[3];
-    };
-    // compensated difference: hi parts nearly cancel exactly, the small
-    // parts carry the low-order bits
-    __device__ static float diff(const Pos &a, const Pos &b, int l) { return (a.h[l] - b.h[l]) + (a.s[l] - b.s[l]); }
-    __device__ static double dbl(const Pos &a, int l) { return (double)a.h[l] + (double)a.s[l]; }
-};
+__device__ __forceinline__ double pdiff(const Pos &a, const Pos &b, int l) { return a.x[l] - b.x[l]; }
 
-
-// FP32 Stokes current from the six face neighbours, positions and fields
-// taken relative to the point itself (r_v = x_v - x_c, b_v = B_v - B_c).
-// For a closed quadrilateral v0..v3 the reference's circulation
-// sum_e (b_e+1 + b_e)/2 . (x_e+1 - x_e) equals 1/2 sum_e (b_e . r_e+1 - b_e+1 . r_e)
-// and its mean area vector 1/4 sum_e d_e x d_e+1 equals 1/2 (r2 - r0) x (r3 - r1);
-// both identities are exact in real arithmetic and take half the operations.
-// Order of the vertices as src/current.cu:291-382: xy W S E N, yz y- z- y+ z+,
-// xz z- x- z+ x+ (indices into xm, ym, xp, yp, zm, zp = 0..5).
-__device__ __forceinline__ void stokes_f32(const float r[6][3], const float b[6][3], float J[3])
-{
-#pragma clang fp contract(fast)
-    const int Q[3][4] = {{0, 1, 2, 3}, {1, 4, 3, 5}, {4, 0, 5, 2}};
-    float NA[3][3], II[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        float I = 0;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const float *be = b[Q[q][e]], *bn = b[Q[q][(e + 1) & 3]];
-            const float *re = r[Q[q][e]], *rn = r[Q[q][(e + 1) & 3]];
-            I += (be[0] * rn[0] + be[1] * rn[1] + be[2] * rn[2]) - (bn[0] * re[0] + bn[1] * re[1] + bn[2] * re[2]);
-        }
-        II[q] = 0.5f * I;
-        const float *r0 = r[Q[q][0]], *r1 = r[Q[q][1]], *r2 = r[Q[q][2]], *r3 = r[Q[q][3]];
-        float u[3] = {r2[0] - r0[0], r2[1] - r0[1], r2[2] - r0[2]};
-        float v[3] = {r3[0] - r1[0], r3[1] - r1[1], r3[2] - r1[2]};
-        NA[q][0] = 0.5f * (u[1] * v[2] - u[2] * v[1]);
-        NA[q][1] = 0.5f * (u[2] * v[0] - u[0] * v[2]);
-        NA[q][2] = 0.5f * (u[0] * v[1] - u[1] * v[0]);
-    }
-    // J = NA^-1 II by the adjugate
-    float A00 = NA[1][1] * NA[2][2] - NA[1][2] * NA[2][1];
-    float A01 = NA[0][2] * NA[2][1] - NA[0][1] * NA[2][2];
-    float A02 = NA[0][1] * NA[1][2] - NA[0][2] * NA[1][1];
-    float A10 = NA[1][2] * NA[2][0] - NA[1][0] * NA[2][2];
-    float A11 = NA[0][0] * NA[2][2] - NA[0][2] * NA[2][0];
-    float A12 = NA[0][2] * NA[1][0] - NA[0][0] * NA[1][2];
-    float A20 = NA[1][0] * NA[2][1] - NA[1][1] * NA[2][0];
-    float A21 = NA[0][1] * NA[2][0] - NA[0][0] * NA[2][1];
-    float A22 = NA[0][0] * NA[1][1] - NA[0][1] * NA[1][0];
-    float det = NA[0][0] * A00 + NA[0][1] * A10 + NA[0][2] * A20;
-    float d1 = 1.0f / det;
-    J[0] = (A00 * II[0] + A01 * II[1] + A02 * II[2]) * d1;
-    J[1] = (A10 * II[0] + A11 * II[1] + A12 * II[2]) * d1;
-    J[2] = (A20 * II[0] + A21 * II[1] + A22 * II[2]) * d1;
-}
 }  // namespace
 
-// Waves per SIMD a stage kernel is register-allocated for: the FP32 stages
-// 0-2 and the fields-only pass fit 128 VGPRs without spilling (two
-// 512-thread workgroups per CU); the later FP32 stages hold more slopes in
-// flight and, like the FP64 and pressure variants, keep 256 VGPRs.
-constexpr int stage_waves(bool f32, int stage, bool press)
-{
-    return (f32 && !press && (stage <= 2 || stage == 6)) ? 4 : 2;
-}
-
 // =============================================================== kernel
-// F32: FP32 stages with compensated positions (else FP64).  NT: threads =
-// largest rect.  STAGE: 0..5, or 6 for the fields of the current positions
-// only (no slope written).  PRESS: pressure force on.
-template <bool F32, int NT, int STAGE, bool PRESS>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(stage_waves(F32, STAGE, PRESS), 8)))
-k_stage(FusedArgs A)
+// NT: threads = largest rect.  STAGE: 0..5, or 6 for the fields of the
+// current positions only (no slope written).  PRESS: pressure force on.
+template <int NT, int STAGE, bool PRESS>
+__global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
 {
     constexpr bool FINAL = STAGE == 5;
-    using PT = PrecT<F32>;
-    using R = typename PT::R;
-    using Pos = typename PT::Pos;
+    using R = double;
     constexpr int NXL = 5;  // planes of positions kept in LDS (ring)
 
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS rings: positions [NXL][3][NT] (hi and small part for F32), B [3][3][NT]
+    // LDS rings: positions [NXL][3][NT], B [3][3][NT]
     R *XLa = reinterpret_cast<R *>(smem);
-    R *XLb = XLa + (F32 ? NXL * 3 * NT : 0);
-    R *BL = XLa + (F32 ? 2 : 1) * NXL * 3 * NT;
+    R *BL = XLa + NXL * 3 * NT;
     __shared__ double red[7][NT / 64];
 
     const Geo &g = A.g;
@@ -303,10 +180,9 @@ k_stage(FusedArgs A)
     const int cface = gx ? 0 : 1, cside = gx ? (gi != 0) : (gj != 0);
     const int cu = gx ? jj : ii;
     const double eps_xy = (g.epsilonProf && inJ) ? A.epsx[ii] * A.epsy[jj] : 1.0;
-    const R *Xh0 = F32 ? (const R *)A.Xh[cur] : (const R *)A.X64[cur];
-    const R *Xl0 = F32 ? (const R *)A.Xl[cur] : nullptr;
-    const R *B0 = F32 ? (const R *)A.B0_32 : (const R *)A.B0_64;
-    const R *K = F32 ? (const R *)A.K32 : (const R *)A.K64;
+    const R *Xh0 = A.X64[cur];
+    const R *B0 = A.B0_64;
+    const R *K = A.K64;
 
     // ---- register state of the column
     Pos xq[4];       // x_n at planes p-2 .. p+1
@@ -315,12 +191,8 @@ k_stage(FusedArgs A)
     R b0q[2][3];     // B0 at planes p, p+1
     // prefetched inputs of one plane
     struct In {
-        R x[3], lo[3], k[5][3], b0[3];
+        R x[3], k[5][3], b0[3];
     } pf;
-    // F32 final stage: per plane lo word, sum c_m k_m and sum (c_m - cs_m) k_m over m < 5
-    float fl[3][3], fc[3][3];
-    double fe[3][3];
-    (void)fl; (void)fc; (void)fe;
 
     double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     int nan_seen = 0;
@@ -332,26 +204,12 @@ k_stage(FusedArgs A)
     auto ld_x = [&](int q, int pt, Pos &o) {
         int s = XLslot(q);
 #pragma unroll
-        for (int l = 0; l < 3; ++l) {
-            if constexpr (F32) {
-                o.h[l] = XLa[(s * 3 + l) * NT + pt];
-                o.s[l] = XLb[(s * 3 + l) * NT + pt];
-            } else {
-                o.x[l] = XLa[(s * 3 + l) * NT + pt];
-            }
-        }
+        for (int l = 0; l < 3; ++l) o.x[l] = XLa[(s * 3 + l) * NT + pt];
     };
     auto st_x = [&](int q, const Pos &v) {
         int s = XLslot(q);
 #pragma unroll
-        for (int l = 0; l < 3; ++l) {
-            if constexpr (F32) {
-                XLa[(s * 3 + l) * NT + t] = v.h[l];
-                XLb[(s * 3 + l) * NT + t] = v.s[l];
-            } else {
-                XLa[(s * 3 + l) * NT + t] = v.x[l];
-            }
-        }
+        for (int l = 0; l < 3; ++l) XLa[(s * 3 + l) * NT + t] = v.x[l];
     };
     auto ld_b = [&](int q, int pt, R o[3]) {
         int s = (unsigned)(q + 12) % 3u;
@@ -366,7 +224,6 @@ k_stage(FusedArgs A)
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
             in.x[l] = Xh0[l * P + a];
-            if constexpr (F32) in.lo[l] = Xl0[l * P + a];
             in.b0[l] = B0[l * P + a];
         }
         if (!gh) {
@@ -378,42 +235,19 @@ k_stage(FusedArgs A)
         }
     };
     // (1) stage position from the loaded inputs
-    auto form_x = [&](int q, const In &in, Pos &o, int fslot) {
+    auto form_x = [&](int q, const In &in, Pos &o) {
         bool gh = ghost_pt(q);
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
-            if constexpr (F32) {
-                float s = in.lo[l];
-                if (!gh)
+            // x_n by the reference's incremental stage updates
+            double x = in.x[l];
+            if (!gh)
 #pragma unroll
-                    for (int m = 0; m < 5; ++m)
-                        if (m < n) s += (float)c_tab.b[n * 6 + m] * in.k[m][l];
-                o.h[l] = in.x[l];
-                o.s[l] = s;
-                if constexpr (FINAL) {
-                    float dc = 0;
-                    double er = 0;
-                    if (!gh)
+                for (int s2 = 0; s2 < 5; ++s2)
+                    if (s2 < n)
 #pragma unroll
-                        for (int m = 0; m < 5; ++m) {
-                            dc += (float)c_tab.c[m] * in.k[m][l];
-                            er += c_tab.ce[m] * (double)in.k[m][l];
-                        }
-                    fl[fslot][l] = in.lo[l];
-                    fc[fslot][l] = dc;
-                    fe[fslot][l] = er;
-                }
-            } else {
-                // x_n by the reference's incremental stage updates
-                double x = in.x[l];
-                if (!gh)
-#pragma unroll
-                    for (int s2 = 0; s2 < 5; ++s2)
-                        if (s2 < n)
-#pragma unroll
-                            for (int m = 0; m <= s2; ++m) x += c_tab.db[s2 * 6 + m] * in.k[m][l];
-                o.x[l] = x;
-            }
+                        for (int m = 0; m <= s2; ++m) x += c_tab.db[s2 * 6 + m] * in.k[m][l];
+            o.x[l] = x;
         }
     };
 
@@ -422,15 +256,17 @@ k_stage(FusedArgs A)
     for (int p = p_first; p <= p_last; ++p) {
         // (1) x_n(p+1) from the inputs loaded one iteration ago
         if (col && plane_ok(p + 1)) {
-            form_x(p + 1, pf, xq[3], 2);
+            form_x(p + 1, pf, xq[3]);
             st_x(p + 1, xq[3]);
 #pragma unroll
             for (int l = 0; l < 3; ++l) b0q[1][l] = pf.b0[l];
         }
         // (0) loads of plane p+2, in flight while B and J are computed
         if (col && plane_ok(p + 2)) load_plane(p + 2, pf);
-        // (2) B(p)
-        if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1) {
+        // (2) B(p); ghost edges (two ghost coordinates) keep B0, as in the
+        // reference, and are never read by J
+        if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1 &&
+            !((gx || gy) && (p == 0 || p == g.nz + 1))) {
             const bool zg = p == 0 || p == g.nz + 1;
             R Bv[3];
             double detv = 0;
@@ -442,29 +278,16 @@ k_stage(FusedArgs A)
                 ld_x(p, t + RW, yp);
                 R jac[3][3], d[3];
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d[l] = PT::diff(xp, xm, l);
+                for (int l = 0; l < 3; ++l) d[l] = pdiff(xp, xm, l);
                 jac_col(jac, 0, d, (R)g.dx1);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d[l] = PT::diff(yp, ym, l);
+                for (int l = 0; l < 3; ++l) d[l] = pdiff(yp, ym, l);
                 jac_col(jac, 1, d, (R)g.dy1);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d[l] = PT::diff(xq[3], xq[1], l);
+                for (int l = 0; l < 3; ++l) d[l] = pdiff(xq[3], xq[1], l);
                 jac_col(jac, 2, d, (R)g.dz1);
-                if constexpr (F32) {
-                    double jd[3][3];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-#pragma unroll
-                        for (int b = 0; b < 3; ++b) jd[a][b] = jac[a][b];
-                    detv = det3(jd);
-                    float rd = 1.0f / (float)detv;
-#pragma unroll
-                    for (int l = 0; l < 3; ++l)
-                        Bv[l] = (jac[l][0] * b0q[0][0] + jac[l][1] * b0q[0][1] + jac[l][2] * b0q[0][2]) * rd;
-                } else {
-                    double detJac1 = cauchy_b(jac, b0q[0], -1, Bv);
-                    detv = 1. / detJac1;
-                }
+                double detJac1 = cauchy_b(jac, b0q[0], -1, Bv);
+                detv = 1. / detJac1;
             } else {
                 // single ghost coordinate: line-tied face formula
                 int face, side, u, v;
@@ -476,7 +299,7 @@ k_stage(FusedArgs A)
                     v = jj;
                     const Pos &pin = side ? xq[1] : xq[3];
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) xin[l] = PT::dbl(pin, l);
+                    for (int l = 0; l < 3; ++l) xin[l] = pin.x[l];
                 } else {
                     face = cface;
                     side = cside;
@@ -486,7 +309,7 @@ k_stage(FusedArgs A)
                     Pos pin;
                     ld_x(p, t + off, pin);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) xin[l] = PT::dbl(pin, l);
+                    for (int l = 0; l < 3; ++l) xin[l] = pin.x[l];
                 }
                 const GhostPt &G = A.ghost[A.ghost_off[face * 2 + side] + u + (long long)nface(g, face) * v];
                 double Bd[3];
@@ -498,10 +321,7 @@ k_stage(FusedArgs A)
                     if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
                         long long a = abase + g.sxy * (long long)p;
 #pragma unroll
-                        for (int l = 0; l < 3; ++l) {
-                            if constexpr (F32) A.eB32[l * P + a] = Bv[l];
-                            else A.eB64[l * P + a] = Bv[l];
-                        }
+                        for (int l = 0; l < 3; ++l) A.eB64[l * P + a] = Bv[l];
                         A.edet[a] = detv;
                     }
                 }
@@ -534,20 +354,7 @@ k_stage(FusedArgs A)
             ld_b(pj, t + RW, Byp);
             const R *Bzm = bq[0], *Bzp = bq[2], *Bc = bq[1];
             R Jv[3], kv[3];
-            if constexpr (F32) {
-                const Pos &xc = xq[1];
-                const Pos *V[6] = {&xm, &ym, &xp, &yp, &zm, &zp};
-                const float *W[6] = {Bxm, Bym, Bxp, Byp, Bzm, Bzp};
-                float rr[6][3], bb[6][3];
-#pragma unroll
-                for (int v = 0; v < 6; ++v)
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) {
-                        rr[v][l] = PT::diff(*V[v], xc, l);
-                        bb[v][l] = W[v][l] - Bc[l];
-                    }
-                stokes_f32(rr, bb, Jv);
-            } else {
+            {
                 // diamonds in circulation order (src/current.cu:291-382):
                 // xy: W S E N; yz: y- z- y+ z+; xz: z- x- z+ x+
                 const Pos *V[3][4] = {{&xm, &ym, &xp, &yp}, {&ym, &zm, &yp, &zp}, {&zm, &xm, &zp, &xp}};
@@ -560,7 +367,7 @@ k_stage(FusedArgs A)
                     for (int e = 0; e < 4; ++e)
 #pragma unroll
                         for (int l = 0; l < 3; ++l) {
-                            d[e][l] = PT::diff(*V[q][(e + 1) & 3], *V[q][e], l);
+                            d[e][l] = pdiff(*V[q][(e + 1) & 3], *V[q][e], l);
                             bv[e][l] = W[q][e][l];
                         }
                     II[q] = diamond(d, bv, NA[q]);
@@ -585,13 +392,13 @@ k_stage(FusedArgs A)
                 Pos pa, pb, pc, pd;
                 R jac[3][3], d2[3][3][3], d[3];
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d[l] = PT::diff(xp, xm, l);
+                for (int l = 0; l < 3; ++l) d[l] = pdiff(xp, xm, l);
                 jac_col(jac, 0, d, (R)g.dx1);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d[l] = PT::diff(yp, ym, l);
+                for (int l = 0; l < 3; ++l) d[l] = pdiff(yp, ym, l);
                 jac_col(jac, 1, d, (R)g.dy1);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d[l] = PT::diff(zp, zm, l);
+                for (int l = 0; l < 3; ++l) d[l] = pdiff(zp, zm, l);
                 jac_col(jac, 2, d, (R)g.dz1);
                 const Pos &x0c = xq[1];
                 const R h1[3] = {(R)g.dx1, (R)g.dy1, (R)g.dz1};
@@ -599,12 +406,8 @@ k_stage(FusedArgs A)
 #pragma unroll
                 for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) {
-                        if constexpr (F32)
-                            d2[l][a2][a2] = (PT::diff(*ax[a2][1], x0c, l) - PT::diff(x0c, *ax[a2][0], l)) * h1[a2] * h1[a2];
-                        else
-                            d2[l][a2][a2] = (ax[a2][1]->x[l] - 2 * x0c.x[l] + ax[a2][0]->x[l]) * h1[a2] * h1[a2];
-                    }
+                    for (int l = 0; l < 3; ++l)
+                        d2[l][a2][a2] = (ax[a2][1]->x[l] - 2 * x0c.x[l] + ax[a2][0]->x[l]) * h1[a2] * h1[a2];
                 // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b)) for (a,b) = (0,1),(0,2),(1,2)
                 const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
 #pragma unroll
@@ -620,9 +423,7 @@ k_stage(FusedArgs A)
 #pragma unroll
                     for (int l = 0; l < 3; ++l) {
                         R v;
-                        if constexpr (F32)
-                            v = (PT::diff(pa, pb, l) - PT::diff(pc, pd, l)) * h1[aa] * h1[bb] / 4;
-                        else if (m == 2)  // the reference's term order for d2/dy dz
+                        if (m == 2)  // the reference's term order for d2/dy dz
                             v = (pa.x[l] - pc.x[l] - pb.x[l] + pd.x[l]) * h1[aa] * h1[bb] / 4;
                         else
                             v = (pa.x[l] - pb.x[l] - pc.x[l] + pd.x[l]) * h1[aa] * h1[bb] / 4;
@@ -641,15 +442,9 @@ k_stage(FusedArgs A)
             if (A.emit) {
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
-                    if constexpr (F32) {
-                        A.eB32[l * P + a] = Bc[l];
-                        A.eJ32[l * P + a] = Jv[l];
-                        if (PRESS) A.eG32[l * P + a] = gv[l];
-                    } else {
-                        A.eB64[l * P + a] = Bc[l];
-                        A.eJ64[l * P + a] = Jv[l];
-                        if (PRESS) A.eG64[l * P + a] = gv[l];
-                    }
+                    A.eB64[l * P + a] = Bc[l];
+                    A.eJ64[l * P + a] = Jv[l];
+                    if (PRESS) A.eG64[l * P + a] = gv[l];
                 }
                 A.edet[a] = detv;
             }
@@ -673,31 +468,9 @@ k_stage(FusedArgs A)
             }
             if constexpr (!FINAL) {
                 if constexpr (stage >= 0) {
-                    R *Ko = F32 ? (R *)A.K32 : (R *)A.K64;
+                    R *Ko = A.K64;
 #pragma unroll
                     for (int l = 0; l < 3; ++l) Ko[stage * P3 + l * P + a] = kv[l];
-                }
-            } else if constexpr (F32) {
-                float *Xhd = A.Xh[cur ^ 1], *Xld = A.Xl[cur ^ 1];
-#pragma unroll
-                for (int l = 0; l < 3; ++l) {
-                    float hi = xq[1].h[l], lo = fl[0][l];
-                    if (l < 2 || g.zUpdate) {
-                        float dC = fc[0][l] + (float)c_tab.c[5] * kv[l];
-                        double err = fe[0][l] + c_tab.ce[5] * (double)kv[l];
-                        // (hi, lo) + dC, keeping the rounding error of hi + dC
-                        float s, e;
-                        two_sum(hi, dC, s, e);
-                        float lo2 = lo + e;
-                        float hi2 = s + lo2;
-                        lo = lo2 - (hi2 - s);
-                        hi = hi2;
-                        double dd = fabs(err);
-                        if (dd != dd) nan_seen = 1;
-                        dmax = nb_max(dmax, dd);
-                    }
-                    Xhd[l * P + a] = hi;
-                    Xld[l * P + a] = lo;
                 }
             } else {
                 // x_6 = x_5 + sum_m (b6m - b5m) k_m, then the x5/x4 weights
@@ -744,14 +517,6 @@ k_stage(FusedArgs A)
             b0q[0][l] = b0q[1][l];
         }
         dq[0] = dq[1];
-        if constexpr (FINAL && F32) {
-#pragma unroll
-            for (int l = 0; l < 3; ++l) {
-                fl[0][l] = fl[1][l]; fl[1][l] = fl[2][l];
-                fc[0][l] = fc[1][l]; fc[1][l] = fc[2][l];
-                fe[0][l] = fe[1][l]; fe[1][l] = fe[2][l];
-            }
-        }
     }
 
     // ---- block partials (fixed order: deterministic)
@@ -901,46 +666,50 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
 
 // ================================================================ launch
 namespace {
-constexpr int kNT32 = 512;   // F32K: threads (= largest rect) per workgroup
 constexpr int kNT64 = 512;   // F64
 
-template <bool F32, int NT>
-size_t lds_bytes()
+template <int NT>
+constexpr size_t lds_bytes64()
 {
-    return (size_t)((F32 ? 2 : 1) * 5 * 3 * NT + 3 * 3 * NT) * (F32 ? sizeof(float) : sizeof(double));
+    return (size_t)(5 * 3 * NT + 3 * 3 * NT) * sizeof(double);
 }
 
-template <bool F32, int NT, int STAGE, bool PRESS>
+template <int NT, int STAGE, bool PRESS>
 void launch_one(hipStream_t s, const FusedArgs &A)
 {
     static bool attr = false;
-    size_t lds = lds_bytes<F32, NT>();
+    constexpr size_t lds = lds_bytes64<NT>();
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_stage<F32, NT, STAGE, PRESS>,
+        (void)hipFuncSetAttribute((const void *)k_stage64<NT, STAGE, PRESS>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage<F32, NT, STAGE, PRESS>), dim3(A.nblocks), dim3(NT), lds, s, A);
+    hipLaunchKernelGGL((k_stage64<NT, STAGE, PRESS>), dim3(A.nblocks), dim3(NT), lds, s, A);
 }
 
-template <bool F32, int NT, bool PRESS>
+template <int NT, bool PRESS>
 void launch_stage(hipStream_t s, const FusedArgs &A)
 {
     switch (A.stage) {
-    case 0: launch_one<F32, NT, 0, PRESS>(s, A); break;
-    case 1: launch_one<F32, NT, 1, PRESS>(s, A); break;
-    case 2: launch_one<F32, NT, 2, PRESS>(s, A); break;
-    case 3: launch_one<F32, NT, 3, PRESS>(s, A); break;
-    case 4: launch_one<F32, NT, 4, PRESS>(s, A); break;
-    case 5: launch_one<F32, NT, 5, PRESS>(s, A); break;
-    default: launch_one<F32, NT, 6, PRESS>(s, A); break;
+    case 0: launch_one<NT, 0, PRESS>(s, A); break;
+    case 1: launch_one<NT, 1, PRESS>(s, A); break;
+    case 2: launch_one<NT, 2, PRESS>(s, A); break;
+    case 3: launch_one<NT, 3, PRESS>(s, A); break;
+    case 4: launch_one<NT, 4, PRESS>(s, A); break;
+    case 5: launch_one<NT, 5, PRESS>(s, A); break;
+    default: launch_one<NT, 6, PRESS>(s, A); break;
     }
 }
 }  // namespace
 
-int fused_max_points(int f32k) { return f32k ? kNT32 : kNT64; }
+// both stage kernels hold up to 512 rect points (one per thread)
+int fused_max_points(int) { return kNT64; }
 
-void set_tableau(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof t); }
+void set_tableau(const TableauF &t)
+{
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &t, sizeof t);
+    set_tableau32(t);
+}
 
 long long ghost_points(const Geo &g, long long off[6])
 {
@@ -968,11 +737,10 @@ void launch_fused(hipStream_t s, const FusedArgs &A, int f32k)
 {
     bool pr = A.g.pressure != 0;
     if (f32k) {
-        if (pr) launch_stage<true, kNT32, true>(s, A);
-        else launch_stage<true, kNT32, false>(s, A);
+        launch_fused32(s, A);
     } else {
-        if (pr) launch_stage<false, kNT64, true>(s, A);
-        else launch_stage<false, kNT64, false>(s, A);
+        if (pr) launch_stage<kNT64, true>(s, A);
+        else launch_stage<kNT64, false>(s, A);
     }
 }
 

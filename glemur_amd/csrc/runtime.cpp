@@ -127,6 +127,13 @@ void cash_karp(double b[42], double c[6], double cs[6])
 
 bool any_peri(const gm_params &p) { return p.xPeri || p.yPeri || p.zPeri; }
 
+bool distorted_ghosts(const gm_params &p)
+{
+    std::string d(p.initDist, strnlen(p.initDist, sizeof p.initDist));
+    while (!d.empty() && (d.back() == ' ' || d.back() == '\0')) d.pop_back();
+    return d == "initShearX" || d == "centerShift";
+}
+
 // Tile choice: the (TX+4)(TY+4) rect must fit the compiled LDS capacity;
 // among those, take the smallest loaded points per output point, then
 // split z so that the launch has about `target` workgroups.
@@ -160,11 +167,12 @@ void auto_tiles(gm_ctx *c, int target)
     c->nblocks = cols * c->nchunks;
 }
 
-// Workgroups per launch: one round over the CUs at the stage kernels'
-// occupancy (two 512-thread workgroups per CU for the FP32 stages).
+// Workgroups per launch: one round over the CUs, one 512-thread workgroup
+// per CU (measured at 129x129x385: two per CU, with half the z chunk, is 8%
+// slower for F32K and 6% for FP64).
 int default_blocks(const gm_ctx *c)
 {
-    return c->ncu * (c->prec == GM_PREC_F32K ? 2 : 1);
+    return c->ncu;
 }
 
 int ensure_partials(gm_ctx *c)
@@ -186,6 +194,10 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
     A.TX = c->TX; A.TY = c->TY; A.CZ = c->CZ;
     A.tiles_x = c->tiles_x; A.tiles_y = c->tiles_y; A.nchunks = c->nchunks; A.nblocks = c->nblocks;
     A.emit = emit; A.diag = diag; A.check_done = check_done;
+    // an initial distortion (src/init.cu:388-419) moves the x/y ghost
+    // layers off the lattice, where the reference's tile-edge rebuilding and
+    // the ghost layer itself differ: those runs take every face from the table
+    A.face_table = distorted_ghosts(c->p);
     for (int i = 0; i < 3; ++i) A.X64[i] = c->X64[i];
     A.B0_64 = c->B0_64; A.K64 = c->K64;
     A.eB64 = c->B64; A.eJ64 = c->J64; A.eG64 = c->G64;

@@ -63,6 +63,14 @@ def inner_mask(p):
     return m.reshape(-1)
 
 
+def edge_mask(p):
+    """Ghost points with two or three ghost coordinates (edges, corners)."""
+    k, j, i = np.meshgrid(np.arange(p.nz + 2), np.arange(p.ny + 2), np.arange(p.nx + 2), indexing="ij")
+    g = ((i == 0) | (i == p.nx + 1)).astype(int) + ((j == 0) | (j == p.ny + 1)).astype(int) + \
+        ((k == 0) | (k == p.nz + 1)).astype(int)
+    return (g >= 2).reshape(-1)
+
+
 def face_mask(p):
     """Ghost points with exactly one ghost coordinate (the six faces)."""
     k, j, i = np.meshgrid(np.arange(p.nz + 2), np.arange(p.ny + 2), np.arange(p.nx + 2), indexing="ij")

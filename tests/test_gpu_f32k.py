@@ -12,7 +12,7 @@ and the fused FP64 step equals the per-kernel FP64 path bit for bit.
 import numpy as np
 import pytest
 
-from conftest import evolved_state, face_mask, inner_mask, small_braid
+from conftest import edge_mask, evolved_state, face_mask, inner_mask, small_braid
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +51,10 @@ def test_f32k_fields_accuracy(g, case):
         eJ = relerr(ctx.download(g.FIELD_J), sim.arr("J"))
         print(f"F32K field errors: B {eB:.2e} det {ed:.2e} J {eJ:.2e}")
         assert eB < 2e-6 and ed < 2e-6 and eJ < 5e-6
+        # ghost edges and corners are never written: B stays B0 (in FP32)
+        e = edge_mask(p)
+        want = B0.reshape(-1, 3)[e].astype(np.float32).astype(np.float64)
+        assert np.array_equal(ctx.download(g.FIELD_B).reshape(-1, 3)[e], want)
         if p.pressure:
             sim.gradp()
             eG = relerr(ctx.download(g.FIELD_GRADP), sim.arr("gradP"))

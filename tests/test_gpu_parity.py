@@ -10,7 +10,7 @@ is compared to 2 ulp and the oracle is then given the device's dt.
 import numpy as np
 import pytest
 
-from conftest import evolved_state, face_mask, inner_mask, small_braid
+from conftest import edge_mask, evolved_state, face_mask, inner_mask, small_braid
 
 pytestmark = pytest.mark.gpu
 
@@ -106,6 +106,9 @@ def test_fused_fields_bitwise(g, case):
         if p.pressure:
             sim.gradp()
             assert_bitwise(ctx.download(g.FIELD_GRADP), sim.arr("gradP"), "gradP")
+        # ghost edges and corners are never written: B stays B0 there
+        e = edge_mask(p)
+        assert_bitwise(ctx.download(g.FIELD_B).reshape(-1, 3)[e], B0.reshape(-1, 3)[e], "B on ghost edges")
 
 
 @pytest.mark.parametrize("tile", [None, (5, 3, 4), (7, 6, 23), (13, 2, 1)])
