@@ -158,6 +158,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
     // ---- registers carried between planes
     float b0q[2][3];      // B0 at planes p, p+1
     double dq[2];         // det at planes p-1, p
+    float drq[2][3][3];   // central differences x(+a) - x(-a) at planes p-1, p
     struct In {
         float x[3], lo[3], k[5][3], b0[3];
     } pf;                 // inputs of one plane, loaded one iteration ahead
@@ -235,7 +236,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
                 L.ldx(p - 1, t, xzm);
             }
             if (!gx && !gy && !zg) {
-                // Jacobian column by column, one neighbour pair at a time
+                // central differences of x along each axis, one neighbour
+                // pair at a time; they give the Jacobian here and, kept for
+                // one plane, the current at this point in the next iteration
                 float jac[3][3];
                 const float hx = 0.5f * (float)g.dx1, hy = 0.5f * (float)g.dy1, hz = 0.5f * (float)g.dz1;
                 {
@@ -243,7 +246,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
                     L.ldx(p, t + 1, a);
                     L.ldx(p, t - 1, b);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) jac[l][0] = dif(a, b, l) * hx;
+                    for (int l = 0; l < 3; ++l) drq[1][0][l] = dif(a, b, l);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 {
@@ -251,11 +254,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
                     L.ldx(p, t + RW, a);
                     L.ldx(p, t - RW, b);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) jac[l][1] = dif(a, b, l) * hy;
+                    for (int l = 0; l < 3; ++l) drq[1][1][l] = dif(a, b, l);
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) jac[l][2] = dif(xzp, xzm, l) * hz;
+                for (int l = 0; l < 3; ++l) {
+                    drq[1][2][l] = dif(xzp, xzm, l);
+                    jac[l][0] = drq[1][0][l] * hx;
+                    jac[l][1] = drq[1][1][l] * hy;
+                    jac[l][2] = drq[1][2][l] * hz;
+                }
                 detv = cauchy32(jac, b0q[0], -1, Bv);
             } else if (!zg && !A.face_table) {
                 // x or y face, line-tied: tangential columns from the ghost
@@ -334,32 +342,51 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
         const int pj = p - 1;
         if (inJ && pj >= Ks && pj <= Ke) {
             const long long a = abase + g.sxy * (long long)pj;
-            // neighbours one at a time, each reduced to its offset from the
-            // point (r = x_v - x_c, b = B_v - B_c) before the next is read:
-            // the scheduler fence keeps the register footprint at one
-            // neighbour instead of all six
-            Pt3 xc;
-            float Bc[3];
-            L.ldx(pj, t, xc);
+            // Stokes current from central differences only.  For the
+            // diamond through the face neighbours v0..v3 of the reference
+            // (src/current.cu:291-382) the trapezoid circulation is
+            // (1/2)(db02 . dr13 - db13 . dr02) and the mean area vector
+            // (1/2) dr02 x dr13, with d.. = v(+) - v(-) along an axis.  With
+            // the diamonds xy, yz, zx the 3x3 system has the cofactor
+            // vectors of [dx dy dz] as rows, so its solution is
+            //   J = (C_x x db_x + C_y x db_y + C_z x db_z) / det[dx dy dz],
+            // C_x = dy x dz, C_y = dz x dx, C_z = dx x dy -- identical to the
+            // reference's circulation over area in exact arithmetic, without
+            // reading the six neighbour positions again.
+            float Bc[3], db[3][3];
             L.ldb(pj, t, Bc);
-            float rr[6][3], bb[6][3];
-            const int vq[6] = {pj, pj, pj, pj, pj - 1, pj + 1};
-            const int vt[6] = {t - 1, t - RW, t + 1, t + RW, t, t};   // xm, ym, xp, yp, zm, zp
+            {
+                float bp[3], bm[3];
+                L.ldb(pj, t + 1, bp);
+                L.ldb(pj, t - 1, bm);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                Pt3 xv;
-                float bv[3];
-                L.ldx(vq[q], vt[q], xv);
-                L.ldb(vq[q], vt[q], bv);
+                for (int l = 0; l < 3; ++l) db[0][l] = bp[l] - bm[l];
+                L.ldb(pj, t + RW, bp);
+                L.ldb(pj, t - RW, bm);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) {
-                    rr[q][l] = dif(xv, xc, l);
-                    bb[q][l] = bv[l] - Bc[l];
-                }
-                __builtin_amdgcn_sched_barrier(0);
+                for (int l = 0; l < 3; ++l) db[1][l] = bp[l] - bm[l];
+                L.ldb(pj + 1, t, bp);
+                L.ldb(pj - 1, t, bm);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) db[2][l] = bp[l] - bm[l];
             }
-            float Jv[3], kv[3];
-            stokes_f32(rr, bb, Jv);
+            const float (&dr)[3][3] = drq[0];
+            float Jv[3] = {0.f, 0.f, 0.f}, kv[3];
+#pragma unroll
+            for (int a0 = 0; a0 < 3; ++a0) {
+                const int a1 = (a0 + 1) % 3, a2 = (a0 + 2) % 3;
+                float C[3];
+                cross3(dr[a1], dr[a2], C);
+                Jv[0] += C[1] * db[a0][2] - C[2] * db[a0][1];
+                Jv[1] += C[2] * db[a0][0] - C[0] * db[a0][2];
+                Jv[2] += C[0] * db[a0][1] - C[1] * db[a0][0];
+            }
+            {
+                // det[dx dy dz] = 8 hx hy hz det(jac), det(jac) from FP64
+                const float rD = (float)(1.0 / (dq[0] * (8.0 * g.dx * g.dy * g.dz)));
+#pragma unroll
+                for (int l = 0; l < 3; ++l) Jv[l] *= rD;
+            }
             kv[0] = Jv[1] * Bc[2] - Jv[2] * Bc[1];
             kv[1] = Jv[2] * Bc[0] - Jv[0] * Bc[2];
             kv[2] = Jv[0] * Bc[1] - Jv[1] * Bc[0];
@@ -372,15 +399,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
                 // gradP from second differences (src/gradP.cu:93-154)
                 float jac[3][3], d2[3][3][3];
                 const float h1[3] = {(float)g.dx1, (float)g.dy1, (float)g.dz1};
-                // rr rows: 0 x-, 1 y-, 2 x+, 3 y+, 4 z-, 5 z+ (offsets from x_c)
+                Pt3 xc;
+                L.ldx(pj, t, xc);
+                const int vq[6] = {pj, pj, pj, pj, pj - 1, pj + 1};
+                const int vt[6] = {t - 1, t + 1, t - RW, t + RW, t, t};
 #pragma unroll
-                for (int l = 0; l < 3; ++l) {
-                    jac[l][0] = (rr[2][l] - rr[0][l]) * 0.5f * h1[0];
-                    jac[l][1] = (rr[3][l] - rr[1][l]) * 0.5f * h1[1];
-                    jac[l][2] = (rr[5][l] - rr[4][l]) * 0.5f * h1[2];
-                    d2[l][0][0] = (rr[2][l] + rr[0][l]) * h1[0] * h1[0];
-                    d2[l][1][1] = (rr[3][l] + rr[1][l]) * h1[1] * h1[1];
-                    d2[l][2][2] = (rr[5][l] + rr[4][l]) * h1[2] * h1[2];
+                for (int a2 = 0; a2 < 3; ++a2) {
+                    Pt3 xm, xp;
+                    L.ldx(vq[2 * a2], vt[2 * a2], xm);
+                    L.ldx(vq[2 * a2 + 1], vt[2 * a2 + 1], xp);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) {
+                        jac[l][a2] = dr[a2][l] * 0.5f * h1[a2];
+                        d2[l][a2][a2] = (dif(xp, xc, l) - dif(xc, xm, l)) * h1[a2] * h1[a2];
+                    }
                 }
                 // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b))
                 Pt3 pa, pb, pc, pd;
@@ -442,6 +474,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
                 // x_new = x_5 + sum_m (c_m - b5_m) k_m (k_5 included), into
                 // (hi, lo) by a two-sum; error |sum_m (c_m - c*_m) k_m| in FP64
                 float *Xhd = A.Xh[cur ^ 1], *Xld = A.Xl[cur ^ 1];
+                Pt3 xc;
+                L.ldx(pj, t, xc);
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
                     float hi = xc.h[l], lo;
@@ -471,6 +505,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32
 #pragma unroll
         for (int l = 0; l < 3; ++l) b0q[0][l] = b0q[1][l];
         dq[0] = dq[1];
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) drq[0][a2][l] = drq[1][a2][l];
         if constexpr (FINAL) {
 #pragma unroll
             for (int l = 0; l < 3; ++l) {

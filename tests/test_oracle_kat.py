@@ -376,6 +376,38 @@ def test_jstokes_linear_field_affine_grid(seed):
     np.testing.assert_allclose(inn3(p, J), np.broadcast_to(curl, (p.nz, p.ny, p.nx, 3)), rtol=1e-10, atol=1e-10)
 
 
+def test_jstokes_equals_cofactor_form():
+    """On any grid the reference's diamond circulation over area equals
+    J = (C_x x db_x + C_y x db_y + C_z x db_z) / det[dx dy dz] with d. the
+    central differences x(+a) - x(-a), B(+a) - B(-a) and C_x = dy x dz,
+    C_y = dz x dx, C_z = dx x dy (exact algebra: the trapezoid circulation of
+    a diamond is (db02 . dr13 - db13 . dr02) / 2, its mean area dr02 x dr13 / 2,
+    and the 3x3 system has the cofactor vectors as rows).  Random distorted
+    grid and random B, numpy vs the oracle's reference-order sums."""
+    p = grid(nx=7, ny=6, nz=8)
+    rng = np.random.default_rng(16)
+    x = lattice(p)
+    x[1:-1, 1:-1, 1:-1] += 0.08 * rng.standard_normal(x[1:-1, 1:-1, 1:-1].shape)
+    Bf = rng.standard_normal(x.shape)
+    J = np.zeros(3 * p.ninner)
+    o.lib.or_jstokes(ref(p), o.dp(x.reshape(-1).copy()), o.dp(Bf.reshape(-1).copy()), o.dp(J))
+    c = (slice(1, -1),) * 3
+
+    def d(f, ax):
+        sp = [slice(1, -1)] * 3
+        sm = [slice(1, -1)] * 3
+        sp[2 - ax] = slice(2, None)
+        sm[2 - ax] = slice(0, -2)
+        return f[tuple(sp)] - f[tuple(sm)]
+
+    dr = [d(x, a) for a in range(3)]
+    db = [d(Bf, a) for a in range(3)]
+    C = [np.cross(dr[1], dr[2]), np.cross(dr[2], dr[0]), np.cross(dr[0], dr[1])]
+    D = np.einsum("...l,...l->...", dr[0], C[0])
+    want = sum(np.cross(C[a], db[a]) for a in range(3)) / D[..., None]
+    np.testing.assert_allclose(inn3(p, J), want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+
+
 def test_jstokes_uniform_field_zero():
     p = grid(nx=5, ny=6, nz=4)
     rng = np.random.default_rng(12)
