@@ -53,7 +53,8 @@ struct CtlArgs {
     Ctl *ctl;
     const double *part_delta;
     const double *part_diag;
-    int nblocks, diag, nbuf;          // nbuf: 3 (F64) or 2 (F32K) position buffers
+    int nb_delta, nb_diag;            // workgroups (partials) of the final stage and of stage 0
+    int diag, nbuf;                   // nbuf: 3 (F64) or 2 (F32K) position buffers
     double maxError, dtMin, cell;     // cell = dx dy dz
     StepRecord *log;
     int log_cap;
@@ -70,8 +71,10 @@ void launch_fused(hipStream_t s, const FusedArgs &A, int f32k);
 // FP32 stage kernel with compensated positions (kernels_f32.hip).
 void set_tableau32(const TableauF &t);
 void launch_fused32(hipStream_t s, const FusedArgs &A);
-// Workgroup size the FP32 kernel uses for a rect of `rect` points.
+// Workgroup size the FP32 kernel uses for a rect of `rect` points, and the
+// workgroups per CU its stage kernel is built for (stage -1: fields only).
 int threads32(int rect);
+int blocks_per_cu32(int rect, int stage, int press);
 void launch_controller(hipStream_t s, const CtlArgs &A);
 
 }  // namespace gmd

@@ -103,13 +103,20 @@ __device__ __forceinline__ double cauchy32(const float jac[3][3], const float b0
 
 }  // namespace
 
-// Waves per SIMD each workgroup size is register-allocated for: two
-// workgroups per CU (LDS allows two at 512 threads), i.e. 128 VGPRs at 256
-// threads, 168 at 384; 512-thread workgroups keep the compiler's choice.
-constexpr int waves32(int nt) { return nt <= 256 ? 4 : nt <= 384 ? 3 : 2; }
+// Waves per SIMD a stage kernel is register-allocated for.  At 512 threads
+// the stages without pressure other than the final one fit 128 VGPRs without
+// spilling, so two workgroups share a CU (4 waves per SIMD); the final stage
+// (five slopes in flight plus its per-plane sums) and the pressure variants
+// keep the compiler's allocation and one workgroup per CU.  Smaller
+// workgroups: two per CU.
+constexpr int waves32(int nt, int stage, bool press)
+{
+    return nt <= 256 ? 4 : nt <= 384 ? 3 : (!press && stage != 5) ? 4 : 2;
+}
 
 template <int NT, int STAGE, bool PRESS>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT), 8))) k_stage32(FusedArgs A)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT, STAGE, PRESS), 8)))
+k_stage32(FusedArgs A)
 {
 #pragma clang fp contract(fast)
     constexpr bool FINAL = STAGE == 5;
@@ -595,6 +602,11 @@ void launch_nt32(hipStream_t s, const FusedArgs &A)
 }  // namespace
 
 int threads32(int rect) { return rect <= 256 ? 256 : rect <= 384 ? 384 : 512; }
+
+int blocks_per_cu32(int rect, int stage, int press)
+{
+    return threads32(rect) < 512 || waves32(512, stage < 0 ? 6 : stage, press != 0) == 4 ? 2 : 1;
+}
 
 void set_tableau32(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &t, sizeof t); }
 

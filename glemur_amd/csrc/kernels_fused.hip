@@ -569,12 +569,12 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
     __shared__ double red[7][16];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
-    const int nb = A.nblocks;
+    const int nb = A.nb_diag;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double m = 0, e = 0, jm = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-        m = nb_max(m, A.part_delta[i]);
-        if (A.diag) {
+    for (int i = threadIdx.x; i < A.nb_delta; i += blockDim.x) m = nb_max(m, A.part_delta[i]);
+    if (A.diag)
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
             e += A.part_diag[i];
             jm = nb_max(jm, A.part_diag[nb + i]);
             r1 = nb_max(r1, A.part_diag[2 * nb + i]);
@@ -582,7 +582,6 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
             r3 = nb_max(r3, A.part_diag[4 * nb + i]);
             r4 = nb_max(r4, A.part_diag[5 * nb + i]);
         }
-    }
     m = wave_max(m);
     e = wave_sum(e);
     jm = wave_max(jm);

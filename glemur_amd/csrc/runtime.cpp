@@ -55,7 +55,10 @@ struct gm_ctx {
     int *nanflag = nullptr;
     int cur = 0;                  // host copy of ctl->cur (valid after syncs)
     // tiles
-    int TX = 0, TY = 0, CZ = 0, tiles_x = 0, tiles_y = 0, nchunks = 0, nblocks = 0;
+    int TX = 0, TY = 0, tiles_x = 0, tiles_y = 0;
+    // z chunking per stage kernel (index 0..5 the stages, 6 the fields-only
+    // pass): stages built for two workgroups per CU get twice as many
+    int CZ[7] = {}, nchunks[7] = {}, nblocks[7] = {};
     int ncu = 256;
     // timing
     bool timing = false;
@@ -134,6 +137,20 @@ bool distorted_ghosts(const gm_params &p)
     return d == "initShearX" || d == "centerShift";
 }
 
+// Workgroups per CU a stage kernel is built for (stage -1: fields only).
+int stage_blocks_per_cu(const gm_ctx *c, int stage)
+{
+    if (c->prec != GM_PREC_F32K) return 1;
+    return gmd::blocks_per_cu32((c->TX + 4) * (c->TY + 4), stage, c->p.pressure);
+}
+
+void set_chunks(gm_ctx *c, int k, int cz)
+{
+    c->CZ[k] = std::max(1, std::min(cz, c->p.nz));
+    c->nchunks[k] = (c->p.nz + c->CZ[k] - 1) / c->CZ[k];
+    c->nblocks[k] = c->tiles_x * c->tiles_y * c->nchunks[k];
+}
+
 // Tile choice: the (TX+4)(TY+4) rect must fit the compiled LDS capacity;
 // among those, take the smallest loaded points per output point, then
 // split z so that the launch has about `target` workgroups.
@@ -158,18 +175,19 @@ void auto_tiles(gm_ctx *c, int target)
     c->tiles_x = (nx + bx - 1) / bx;
     c->tiles_y = (ny + by - 1) / by;
     int cols = c->tiles_x * c->tiles_y;
-    // z chunks: as many workgroups as the CUs hold in one or more whole
-    // rounds (target = CUs x rounds), never chunks thinner than 8 planes
-    int nch = std::max(1, target / cols);
-    nch = std::min(nch, std::max(1, nz / 8));
-    c->CZ = (nz + nch - 1) / nch;
-    c->nchunks = (nz + c->CZ - 1) / c->CZ;
-    c->nblocks = cols * c->nchunks;
+    // z chunks per stage: as many workgroups as one round over the CUs
+    // holds at that stage kernel's occupancy (target = CUs x workgroups per
+    // CU), never chunks thinner than 8 planes
+    for (int k = 0; k < 7; ++k) {
+        int bpc = stage_blocks_per_cu(c, k == 6 ? -1 : k);
+        int nch = std::max(1, target * bpc / cols);
+        nch = std::min(nch, std::max(1, nz / 8));
+        set_chunks(c, k, (nz + nch - 1) / nch);
+    }
 }
 
-// Workgroups per launch: one round over the CUs, one 512-thread workgroup
-// per CU (measured at 129x129x385: two per CU, with half the z chunk, is 8%
-// slower for F32K and 6% for FP64).
+// Workgroup rounds per launch: one round over the CUs (at each stage
+// kernel's own occupancy).
 int default_blocks(const gm_ctx *c)
 {
     return c->ncu;
@@ -177,12 +195,14 @@ int default_blocks(const gm_ctx *c)
 
 int ensure_partials(gm_ctx *c)
 {
-    if (c->nblocks <= c->part_cap) return GM_OK;
+    int need = 0;
+    for (int k = 0; k < 7; ++k) need = std::max(need, c->nblocks[k]);
+    if (need <= c->part_cap) return GM_OK;
     dfree(c->part_delta);
     dfree(c->part_diag);
     int r;
-    if ((r = dalloc(&c->part_delta, (size_t)c->nblocks)) || (r = dalloc(&c->part_diag, (size_t)6 * c->nblocks))) return r;
-    c->part_cap = c->nblocks;
+    if ((r = dalloc(&c->part_delta, (size_t)need)) || (r = dalloc(&c->part_diag, (size_t)6 * need))) return r;
+    c->part_cap = need;
     return GM_OK;
 }
 
@@ -191,8 +211,9 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
     gmd::FusedArgs A{};
     A.g = c->geo;
     A.stage = stage;
-    A.TX = c->TX; A.TY = c->TY; A.CZ = c->CZ;
-    A.tiles_x = c->tiles_x; A.tiles_y = c->tiles_y; A.nchunks = c->nchunks; A.nblocks = c->nblocks;
+    const int k = stage < 0 ? 6 : stage;
+    A.TX = c->TX; A.TY = c->TY; A.CZ = c->CZ[k];
+    A.tiles_x = c->tiles_x; A.tiles_y = c->tiles_y; A.nchunks = c->nchunks[k]; A.nblocks = c->nblocks[k];
     A.emit = emit; A.diag = diag; A.check_done = check_done;
     // an initial distortion (src/init.cu:388-419) moves the x/y ghost
     // layers off the lattice, where the reference's tile-edge rebuilding and
@@ -261,7 +282,8 @@ void enqueue_step(gm_ctx *c, int diag, int check_done)
     C.ctl = c->ctl;
     C.part_delta = c->part_delta;
     C.part_diag = c->part_diag;
-    C.nblocks = c->nblocks;
+    C.nb_delta = c->nblocks[5];
+    C.nb_diag = c->nblocks[0];
     C.diag = diag;
     C.nbuf = f32k ? 2 : 3;
     C.maxError = c->p.maxError;
@@ -825,6 +847,7 @@ extern "C" double gm_algorithmic_bytes_per_point_stage(int precision)
 extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (tx <= 0 || ty <= 0) {
         auto_tiles(c, cz > 0 ? cz : default_blocks(c));
         return ensure_partials(c);
@@ -835,10 +858,7 @@ extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
     c->TY = std::min(ty, c->p.ny);
     c->tiles_x = (c->p.nx + c->TX - 1) / c->TX;
     c->tiles_y = (c->p.ny + c->TY - 1) / c->TY;
-    c->CZ = std::max(1, std::min(cz > 0 ? cz : c->p.nz, c->p.nz));
-    c->nchunks = (c->p.nz + c->CZ - 1) / c->CZ;
-    c->nblocks = c->tiles_x * c->tiles_y * c->nchunks;
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    for (int k = 0; k < 7; ++k) set_chunks(c, k, cz > 0 ? cz : c->p.nz);
     return ensure_partials(c);
 }
 extern "C" int gm_get_tile(gm_ctx *c, int *tx, int *ty, int *cz, int *nblocks)
@@ -846,8 +866,8 @@ extern "C" int gm_get_tile(gm_ctx *c, int *tx, int *ty, int *cz, int *nblocks)
     if (!c) return fail(GM_ERR_ARG, "null context");
     if (tx) *tx = c->TX;
     if (ty) *ty = c->TY;
-    if (cz) *cz = c->CZ;
-    if (nblocks) *nblocks = c->nblocks;
+    if (cz) *cz = c->CZ[1];           // stage 1 (the common configuration)
+    if (nblocks) *nblocks = c->nblocks[1];
     return GM_OK;
 }
 

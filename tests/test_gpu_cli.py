@@ -1,0 +1,91 @@
+"""The glemur executable under the reference's run-directory contract
+(src/main.cu:44-394, src/io.cu): params.in and data/ in the working
+directory; cuda.info, data/B0.vtk, data/dump<k>.vtk, data/save.vtk and
+data/time_series.dat written; fRestart = t resumes from save.vtk."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "glemur_amd", "glemur")
+
+PARAMS = """&comp
+    nx = 17, ny = 19, nz = 23
+/
+&start
+    Lx = 4, Ly = 4.5, Lz = 12, Ox = -2, Oy = -2.25, Oz = -6
+    bInit = "blobs", ampl = 1, nBlobs = 6
+    blobXc = 0.5, -0.5, 0.5, -0.5, 0.5, -0.5
+    blobYc = 6*0
+    blobZc = -5, -3, -1, 1, 3, 5
+    blobScale = 6*0.8, blobZl = 6*1
+    blobTwist = 0.4, -0.4, 0.4, -0.4, 0.4, -0.4
+    fRestart = {restart}
+/
+&run
+    nCycle = {ncycle}, maxError = 1e-5, dt0 = 3e-4, dtMin = 1e-9
+    epsilonProf = f, pressure = f, zUpdate = t, jMethod = "Stokes"
+/
+&io
+    nTs = 2, dtDump = 0.05, dtSave = 0.1
+    dumpJ = t, dumpDetJac = t
+    redB2 = t, redJMax = t, redJxB_B2Max = t
+/
+"""
+
+
+def run_cli(d, *args, restart=False, ncycle=30):
+    (d / "params.in").write_text(PARAMS.format(restart="t" if restart else "f", ncycle=ncycle))
+    os.makedirs(d / "data", exist_ok=True)
+    print("running glemur", args, "in", d, flush=True)
+    r = subprocess.run([EXE, *args], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def ts_rows(d):
+    lines = (d / "data" / "time_series.dat").read_text().splitlines()
+    assert lines[0].startswith("#")
+    return lines[0].split(), [l.split() for l in lines[1:]]
+
+
+@pytest.mark.parametrize("prec", ["64", "32"])
+def test_run_directory_contract(require_gpu, tmp_path, prec):
+    out = run_cli(tmp_path, "-d", "0", "-p", prec)
+    assert "glemur terminated successfully" in out
+    assert (tmp_path / "cuda.info").stat().st_size > 0
+    assert (tmp_path / "data" / "B0.vtk").exists()
+    assert (tmp_path / "data" / "save.vtk").exists()
+    dumps = [f for f in os.listdir(tmp_path / "data") if f.startswith("dump")]
+    assert dumps
+    head, rows = ts_rows(tmp_path)
+    assert head[:5] == ["#", "it", "t", "dt", "maxDelta"] or head[:4] == ["#it", "t", "dt", "maxDelta"]
+    # nTs = 2: a row on every second accepted step (time index 0, 2, ...)
+    its = [int(r[0]) for r in rows]
+    assert its == list(range(0, 30, 2))
+    t = [float(r[1]) for r in rows]
+    assert all(b > a for a, b in zip(t, t[1:]))
+    b2 = [float(r[head.index("B2") - (1 if head[0] == "#" else 0)]) for r in rows]
+    assert all(b <= a for a, b in zip(b2, b2[1:]))
+
+
+def test_restart_continues(require_gpu, tmp_path):
+    """A run of 30 steps, then fRestart = t for 10 more: the restarted run
+    starts from save.vtk's time and keeps the time series going."""
+    run_cli(tmp_path, "-p", "64", ncycle=30)
+    import glemur_amd as g
+    p = g.params_read(tmp_path / "params.in")
+    xb, B0, t_save, dt_save = g.read_state(tmp_path, p, 64)
+    assert t_save > 0 and dt_save > 0
+    _, rows_before = ts_rows(tmp_path)
+    run_cli(tmp_path, "-p", "64", restart=True, ncycle=10)
+    _, rows_after = ts_rows(tmp_path)
+    assert len(rows_after) > len(rows_before)
+    t_new = [float(r[1]) for r in rows_after[len(rows_before):]]
+    assert min(t_new) > t_save
+    B0_init, _ = g.init_state(p)
+    assert np.array_equal(B0, B0_init)
