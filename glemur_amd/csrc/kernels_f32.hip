@@ -111,8 +111,11 @@ __device__ __forceinline__ double cauchy32(const float jac[3][3], const float b0
 // workgroups: two per CU.
 constexpr int waves32(int nt, int stage, bool press)
 {
-    return nt <= 256 ? 4 : nt <= 384 ? 3 : (!press && stage != 5) ? 4 : 2;
+    return nt <= 256 ? 4 : nt <= 384 ? 3 : nt >= 1024 ? 4 : (!press && stage != 5) ? 4 : 2;
 }
+// 1024-thread workgroups (16 waves, 128 VGPRs) only for the kernels that
+// fit 128 VGPRs without spilling
+constexpr bool fits1024(int stage, bool press) { return !press && stage != 5; }
 
 template <int NT, int STAGE, bool PRESS>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT, STAGE, PRESS), 8)))
@@ -599,13 +602,29 @@ void launch_nt32(hipStream_t s, const FusedArgs &A)
     if (A.g.pressure) launch_stage32<NT, true>(s, A);
     else launch_stage32<NT, false>(s, A);
 }
+// 1024 threads: stages 0-4 and the fields pass without pressure
+void launch_1024(hipStream_t s, const FusedArgs &A)
+{
+    switch (A.stage) {
+    case 0: launch_one32<1024, 0, false>(s, A); break;
+    case 1: launch_one32<1024, 1, false>(s, A); break;
+    case 2: launch_one32<1024, 2, false>(s, A); break;
+    case 3: launch_one32<1024, 3, false>(s, A); break;
+    case 4: launch_one32<1024, 4, false>(s, A); break;
+    default: launch_one32<1024, 6, false>(s, A); break;
+    }
+}
 }  // namespace
 
-int threads32(int rect) { return rect <= 256 ? 256 : rect <= 384 ? 384 : 512; }
+int threads32(int rect) { return rect <= 256 ? 256 : rect <= 384 ? 384 : rect <= 512 ? 512 : 1024; }
+
+int max_rect32(int stage, int press) { return fits1024(stage < 0 ? 6 : stage, press != 0) ? 1024 : 512; }
 
 int blocks_per_cu32(int rect, int stage, int press)
 {
-    return threads32(rect) < 512 || waves32(512, stage < 0 ? 6 : stage, press != 0) == 4 ? 2 : 1;
+    const int nt = threads32(rect);
+    if (nt >= 1024) return 1;
+    return nt < 512 || waves32(512, stage < 0 ? 6 : stage, press != 0) == 4 ? 2 : 1;
 }
 
 void set_tableau32(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &t, sizeof t); }
@@ -616,7 +635,8 @@ void launch_fused32(hipStream_t s, const FusedArgs &A)
     switch (threads32((A.TX + 4) * (A.TY + 4))) {
     case 256: launch_nt32<256>(s, A); break;
     case 384: launch_nt32<384>(s, A); break;
-    default: launch_nt32<512>(s, A); break;
+    case 512: launch_nt32<512>(s, A); break;
+    default: launch_1024(s, A); break;   // the runtime offers 1024 only where fits1024
     }
 }
 

@@ -55,9 +55,10 @@ struct gm_ctx {
     int *nanflag = nullptr;
     int cur = 0;                  // host copy of ctl->cur (valid after syncs)
     // tiles
-    int TX = 0, TY = 0, tiles_x = 0, tiles_y = 0;
-    // z chunking per stage kernel (index 0..5 the stages, 6 the fields-only
-    // pass): stages built for two workgroups per CU get twice as many
+    // tiles per stage kernel (index 0..5 the stages, 6 the fields-only
+    // pass): the stages exchange data through HBM only, so each takes the
+    // rect and z chunking that suit its own workgroup size and occupancy
+    int TX[7] = {}, TY[7] = {}, tiles_x[7] = {}, tiles_y[7] = {};
     int CZ[7] = {}, nchunks[7] = {}, nblocks[7] = {};
     int ncu = 256;
     // timing
@@ -137,52 +138,55 @@ bool distorted_ghosts(const gm_params &p)
     return d == "initShearX" || d == "centerShift";
 }
 
-// Workgroups per CU a stage kernel is built for (stage -1: fields only).
-int stage_blocks_per_cu(const gm_ctx *c, int stage)
+// Workgroups per CU a stage kernel is built for (k: 0..5 stages, 6 fields).
+int stage_blocks_per_cu(const gm_ctx *c, int k)
 {
     if (c->prec != GM_PREC_F32K) return 1;
-    return gmd::blocks_per_cu32((c->TX + 4) * (c->TY + 4), stage, c->p.pressure);
+    return gmd::blocks_per_cu32((c->TX[k] + 4) * (c->TY[k] + 4), k == 6 ? -1 : k, c->p.pressure);
 }
 
-void set_chunks(gm_ctx *c, int k, int cz)
+// Largest rect a stage kernel holds (one point per thread).
+int stage_max_rect(const gm_ctx *c, int k)
 {
+    if (c->prec != GM_PREC_F32K) return gmd::fused_max_points(0);
+    return gmd::max_rect32(k == 6 ? -1 : k, c->p.pressure);
+}
+
+void set_tile_k(gm_ctx *c, int k, int tx, int ty, int cz)
+{
+    c->TX[k] = std::min(tx, c->p.nx);
+    c->TY[k] = std::min(ty, c->p.ny);
+    c->tiles_x[k] = (c->p.nx + c->TX[k] - 1) / c->TX[k];
+    c->tiles_y[k] = (c->p.ny + c->TY[k] - 1) / c->TY[k];
     c->CZ[k] = std::max(1, std::min(cz, c->p.nz));
     c->nchunks[k] = (c->p.nz + c->CZ[k] - 1) / c->CZ[k];
-    c->nblocks[k] = c->tiles_x * c->tiles_y * c->nchunks[k];
+    c->nblocks[k] = c->tiles_x[k] * c->tiles_y[k] * c->nchunks[k];
 }
 
-// Tile choice: the (TX+4)(TY+4) rect must fit the compiled LDS capacity;
-// among those, take the smallest loaded points per output point, then
-// split z so that the launch has about `target` workgroups.
+// Tile choice per stage kernel: among the rects (TX+4)(TY+4) that fit the
+// kernel's workgroup, the fewest loaded points per output point; then z
+// chunks so that one round over the CUs (`target` x workgroups per CU) is
+// filled, never thinner than 8 planes.
 void auto_tiles(gm_ctx *c, int target)
 {
-    const int cap = gmd::fused_max_points(c->prec == GM_PREC_F32K);
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
-    // smallest loaded-points-per-output-point rect (tile plus its 2-point
-    // halo) that fits one workgroup
-    int bx = 4, by = 2;
-    double best = 1e30;
-    for (int ty = 2; ty <= 64; ++ty)
-        for (int txm = 4; (txm + 4) * (ty + 4) <= cap; ++txm) {
-            int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
-            int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
-            if ((tx + 4) * (tyy + 4) > cap) continue;
-            double cost = (double)tlx * tly * (tx + 4) * (tyy + 4) / ((double)nx * ny);
-            if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
-        }
-    c->TX = bx;
-    c->TY = by;
-    c->tiles_x = (nx + bx - 1) / bx;
-    c->tiles_y = (ny + by - 1) / by;
-    int cols = c->tiles_x * c->tiles_y;
-    // z chunks per stage: as many workgroups as one round over the CUs
-    // holds at that stage kernel's occupancy (target = CUs x workgroups per
-    // CU), never chunks thinner than 8 planes
     for (int k = 0; k < 7; ++k) {
-        int bpc = stage_blocks_per_cu(c, k == 6 ? -1 : k);
-        int nch = std::max(1, target * bpc / cols);
+        const int cap = stage_max_rect(c, k);
+        int bx = 4, by = 2;
+        double best = 1e30;
+        for (int ty = 2; ty <= 96; ++ty)
+            for (int txm = 4; (txm + 4) * (ty + 4) <= cap; ++txm) {
+                int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
+                int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
+                if ((tx + 4) * (tyy + 4) > cap) continue;
+                double cost = (double)tlx * tly * (tx + 4) * (tyy + 4) / ((double)nx * ny);
+                if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
+            }
+        set_tile_k(c, k, bx, by, nz);
+        const int cols = c->tiles_x[k] * c->tiles_y[k];
+        int nch = std::max(1, target * stage_blocks_per_cu(c, k) / cols);
         nch = std::min(nch, std::max(1, nz / 8));
-        set_chunks(c, k, (nz + nch - 1) / nch);
+        set_tile_k(c, k, bx, by, (nz + nch - 1) / nch);
     }
 }
 
@@ -212,8 +216,8 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
     A.g = c->geo;
     A.stage = stage;
     const int k = stage < 0 ? 6 : stage;
-    A.TX = c->TX; A.TY = c->TY; A.CZ = c->CZ[k];
-    A.tiles_x = c->tiles_x; A.tiles_y = c->tiles_y; A.nchunks = c->nchunks[k]; A.nblocks = c->nblocks[k];
+    A.TX = c->TX[k]; A.TY = c->TY[k]; A.CZ = c->CZ[k];
+    A.tiles_x = c->tiles_x[k]; A.tiles_y = c->tiles_y[k]; A.nchunks = c->nchunks[k]; A.nblocks = c->nblocks[k];
     A.emit = emit; A.diag = diag; A.check_done = check_done;
     // an initial distortion (src/init.cu:388-419) moves the x/y ghost
     // layers off the lattice, where the reference's tile-edge rebuilding and
@@ -854,18 +858,14 @@ extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
     }
     if ((tx + 4) * (ty + 4) > gmd::fused_max_points(c->prec == GM_PREC_F32K))
         return fail(GM_ERR_ARG, "tile rect (tx+4)(ty+4) exceeds the compiled capacity");
-    c->TX = std::min(tx, c->p.nx);
-    c->TY = std::min(ty, c->p.ny);
-    c->tiles_x = (c->p.nx + c->TX - 1) / c->TX;
-    c->tiles_y = (c->p.ny + c->TY - 1) / c->TY;
-    for (int k = 0; k < 7; ++k) set_chunks(c, k, cz > 0 ? cz : c->p.nz);
+    for (int k = 0; k < 7; ++k) set_tile_k(c, k, tx, ty, cz > 0 ? cz : c->p.nz);
     return ensure_partials(c);
 }
 extern "C" int gm_get_tile(gm_ctx *c, int *tx, int *ty, int *cz, int *nblocks)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
-    if (tx) *tx = c->TX;
-    if (ty) *ty = c->TY;
+    if (tx) *tx = c->TX[1];
+    if (ty) *ty = c->TY[1];
     if (cz) *cz = c->CZ[1];           // stage 1 (the common configuration)
     if (nblocks) *nblocks = c->nblocks[1];
     return GM_OK;
