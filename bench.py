@@ -24,6 +24,10 @@ relaxation: attempted steps, time and force-freeness reached by the F32K
 path in a bounded run from the initial state towards
 max|JxB| / max(|J||B|) <= 1e-3.
 
+ensemble: 64 seeded braids of the sizes 33x33x97 ... 97x97x289 advanced
+together (one launch per stage for all members); aggregate grid-point
+RK-stages per second.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -111,6 +115,26 @@ def relaxation_probe(g, p, B0, xb, seconds):
             "energy": st.energy, "time_to_relax_s": sec if st.converged else None}
 
 
+def ensemble(g, workloads, n, steps):
+    ps = [workloads.ensemble_member(i) for i in range(n)]
+    pts = sum(p.nx * p.ny * p.nz for p in ps)
+    b = g.Batch(ps)
+    for m, p in zip(b.members, ps):
+        B0, xb = g.init_state(p)
+        m.upload(B0, xb, 0.0, 0.01)
+    b.run(max_attempts=3, batch=3, n_cycle=10 ** 9)             # warm-up
+    t0 = time.perf_counter()
+    st = b.run(max_attempts=steps, batch=steps, n_cycle=10 ** 9)
+    sec = time.perf_counter() - t0
+    att = sum(s.n_attempt for s in st) - 3 * n
+    b.close()
+    sizes = sorted({(p.nx, p.ny, p.nz) for p in ps})
+    return {"members": n, "inner_points": pts, "sizes": [list(s) for s in sizes],
+            "attempts_per_member": steps, "seconds": sec,
+            "value": sum(p.nx * p.ny * p.nz * 6 * (s.n_attempt - 3) for p, s in zip(ps, st)) / sec,
+            "unit": "grid-pt RK-stages/s", "attempts_total": int(att)}
+
+
 def cpu_baseline(p, B0, xb, budget_s=20.0):
     import oracle as o
     threads = min(16, os.cpu_count() or 1)
@@ -143,6 +167,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--relax-seconds", type=float, default=10.0)
+    ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -216,6 +241,8 @@ def main():
                       "traffic": traffic_for(traffic, "f64", f64["tile"])}
     if args.relax_seconds > 0:
         out["relaxation"] = relaxation_probe(g, p, B0, xb, args.relax_seconds)
+    if args.ensemble > 0:
+        out["ensemble"] = ensemble(g, workloads, args.ensemble, 20)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, B0, xb)
     print(json.dumps(out))

@@ -157,6 +157,10 @@ _SIGS = {
     "gm_write_ts": (C.c_int, [C.c_char_p, _P(Params), _P(Reduction), C.c_double, C.c_double, C.c_int,
                                C.c_double, C.c_int, C.c_int]),
     "gm_write_device_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int]),
+    "gm_batch_create": (C.c_int, [_P(Params), C.c_int, C.c_int, _P(_CTX)]),
+    "gm_batch_destroy": (C.c_int, [_CTX]),
+    "gm_batch_context": (C.c_int, [_CTX, C.c_int, _P(_CTX)]),
+    "gm_batch_run": (C.c_int, [_CTX, _P(RunOpts), _P(RunStats)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -235,13 +239,24 @@ class Context:
     def __init__(self, params, precision=PREC_F64):
         self.p = params
         self.precision = precision
+        self._owned = True
         self._h = _CTX()
         _check(_lib.gm_create(C.byref(params), precision, C.byref(self._h)))
 
+    @classmethod
+    def _member(cls, handle, params, precision):
+        """A view of an ensemble member (owned by its Batch)."""
+        m = cls.__new__(cls)
+        m.p = params
+        m.precision = precision
+        m._owned = False
+        m._h = handle
+        return m
+
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             _lib.gm_destroy(self._h)
-            self._h = _CTX()
+        self._h = _CTX()
 
     def __del__(self):
         try:
@@ -338,6 +353,54 @@ class Context:
         v = [C.c_int() for _ in range(4)]
         _check(_lib.gm_get_tile(self._h, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+
+class Batch:
+    """An ensemble of relaxations advanced together (gm_batch_*, FP32 with
+    compensated positions).  Members are Context views: upload, download and
+    tiles per member; run() advances all of them."""
+
+    def __init__(self, params_list, precision=PREC_F32K):
+        self.params = list(params_list)
+        arr = (Params * len(self.params))(*self.params)
+        self._h = _CTX()
+        _check(_lib.gm_batch_create(arr, len(self.params), precision, C.byref(self._h)))
+        self.members = []
+        for i, p in enumerate(self.params):
+            h = _CTX()
+            _check(_lib.gm_batch_context(self._h, i, C.byref(h)))
+            self.members.append(Context._member(h, p, precision))
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, i):
+        return self.members[i]
+
+    def run(self, max_attempts=0, n_cycle=0, stop_ratio=0.0, batch=16, use_graph=False):
+        o = RunOpts(max_attempts, n_cycle, stop_ratio, batch, int(use_graph))
+        st = (RunStats * len(self.members))()
+        _check(_lib.gm_batch_run(self._h, C.byref(o), st))
+        return [st[i] for i in range(len(self.members))]
+
+    def close(self):
+        if self._h:
+            for m in self.members:
+                m._h = _CTX()
+            _lib.gm_batch_destroy(self._h)
+            self._h = _CTX()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def write_b0(dirpath, p, B0, precision_bits=64):

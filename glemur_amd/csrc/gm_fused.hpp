@@ -78,5 +78,10 @@ int blocks_per_cu32(int rect, int stage, int press);
 // Largest rect the FP32 kernel of this stage is built for (512 or 1024).
 int max_rect32(int stage, int press);
 void launch_controller(hipStream_t s, const CtlArgs &A);
+// Ensemble launches: every workgroup of n instances in one launch (FP32
+// stage kernel, 512-thread rects), and one controller workgroup per instance.
+void launch_fused32_batch(hipStream_t s, int stage, int press, const FusedArgs *args, const int *start, int n,
+                          int total);
+void launch_controller_batch(hipStream_t s, const CtlArgs *args, int n);
 
 }  // namespace gmd

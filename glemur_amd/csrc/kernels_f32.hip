@@ -117,9 +117,11 @@ constexpr int waves32(int nt, int stage, bool press)
 // fit 128 VGPRs without spilling
 constexpr bool fits1024(int stage, bool press) { return !press && stage != 5; }
 
+// One workgroup's march: `lb` is its logical index in the instance (after
+// the XCD-aware remap), which picks the tile and z chunk and the slot of
+// its partial results.
 template <int NT, int STAGE, bool PRESS>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT, STAGE, PRESS), 8)))
-k_stage32(FusedArgs A)
+__device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 {
 #pragma clang fp contract(fast)
     constexpr bool FINAL = STAGE == 5;
@@ -141,7 +143,7 @@ k_stage32(FusedArgs A)
     const float dt = (float)ctl->dt;
     const long long P = g.P, P3 = 3 * P;
 
-    const int Lb = xcd_remap(blockIdx.x, A.nblocks);
+    const int Lb = lb;
     const int tx = Lb % A.tiles_x, ty = (Lb / A.tiles_x) % A.tiles_y, ch = Lb / (A.tiles_x * A.tiles_y);
     const int ix0 = tx * A.TX, iy0 = ty * A.TY;
     const int Ks = ch * A.CZ + 1, Ke = min(ch * A.CZ + A.CZ, g.nz);
@@ -556,7 +558,7 @@ k_stage32(FusedArgs A)
         if (FINAL) {
             double m = 0;
             for (int q = 0; q < NT / 64; ++q) m = nb_max(m, red[0][q]);
-            A.part_delta[blockIdx.x] = m;
+            A.part_delta[lb] = m;
         }
         if (DIAG && A.diag) {
             double s = 0, am[5] = {0, 0, 0, 0, 0};
@@ -564,10 +566,33 @@ k_stage32(FusedArgs A)
                 s += red[1][q];
                 for (int m = 0; m < 5; ++m) am[m] = nb_max(am[m], red[m + 2][q]);
             }
-            A.part_diag[blockIdx.x] = s;
-            for (int m = 0; m < 5; ++m) A.part_diag[(m + 1) * A.nblocks + blockIdx.x] = am[m];
+            A.part_diag[lb] = s;
+            for (int m = 0; m < 5; ++m) A.part_diag[(m + 1) * A.nblocks + lb] = am[m];
         }
     }
+}
+
+template <int NT, int STAGE, bool PRESS>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT, STAGE, PRESS), 8)))
+k_stage32(FusedArgs A)
+{
+    stage32_body<NT, STAGE, PRESS>(A, xcd_remap(blockIdx.x, A.nblocks));
+}
+
+// Ensemble: one launch covers the workgroups of n instances; start[i] is
+// the first (remapped) workgroup of instance i, start[n] the total.
+template <int NT, int STAGE, bool PRESS>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT, STAGE, PRESS), 8)))
+k_stage32_batch(const FusedArgs *__restrict__ args, const int *__restrict__ start, int n)
+{
+    const int L = xcd_remap(blockIdx.x, start[n]);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {   // last instance whose first workgroup is <= L
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= L) lo = mid;
+        else hi = mid - 1;
+    }
+    stage32_body<NT, STAGE, PRESS>(args[lo], L - start[lo]);
 }
 
 namespace {
@@ -628,6 +653,40 @@ int blocks_per_cu32(int rect, int stage, int press)
 }
 
 void set_tableau32(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &t, sizeof t); }
+
+namespace {
+template <int STAGE, bool PRESS>
+void launch_batch_one(hipStream_t s, const FusedArgs *args, const int *start, int n, int total)
+{
+    static bool attr = false;
+    constexpr size_t lds = lds_bytes32<512>();
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_stage32_batch<512, STAGE, PRESS>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_stage32_batch<512, STAGE, PRESS>), dim3(total), dim3(512), lds, s, args, start, n);
+}
+template <bool PRESS>
+void launch_batch_stage(hipStream_t s, int stage, const FusedArgs *args, const int *start, int n, int total)
+{
+    switch (stage) {
+    case 0: launch_batch_one<0, PRESS>(s, args, start, n, total); break;
+    case 1: launch_batch_one<1, PRESS>(s, args, start, n, total); break;
+    case 2: launch_batch_one<2, PRESS>(s, args, start, n, total); break;
+    case 3: launch_batch_one<3, PRESS>(s, args, start, n, total); break;
+    case 4: launch_batch_one<4, PRESS>(s, args, start, n, total); break;
+    default: launch_batch_one<5, PRESS>(s, args, start, n, total); break;
+    }
+}
+}  // namespace
+
+void launch_fused32_batch(hipStream_t s, int stage, int press, const FusedArgs *args, const int *start, int n,
+                          int total)
+{
+    if (press) launch_batch_stage<true>(s, stage, args, start, n, total);
+    else launch_batch_stage<false>(s, stage, args, start, n, total);
+}
 
 void launch_fused32(hipStream_t s, const FusedArgs &A)
 {

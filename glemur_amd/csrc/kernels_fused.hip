@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
 // ============================================================ controller
 // One workgroup: reduces the stage partials, applies accept/reject and the
 // dt update of src/main.cu:321-383 and logs the attempt.
-__global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
+__device__ __forceinline__ void controller_body(const CtlArgs &A)
 {
     __shared__ double red[7][16];
     Ctl *ctl = A.ctl;
@@ -663,6 +663,14 @@ __global__ void __launch_bounds__(1024) k_controller(CtlArgs A)
     else if (ctl->n_attempt >= ctl->max_attempt) { ctl->done = 1; ctl->stop_reason = 4; }
 }
 
+__global__ void __launch_bounds__(1024) k_controller(CtlArgs A) { controller_body(A); }
+
+// Ensemble: one workgroup per instance.
+__global__ void __launch_bounds__(1024) k_controller_batch(const CtlArgs *__restrict__ args)
+{
+    controller_body(args[blockIdx.x]);
+}
+
 // ================================================================ launch
 namespace {
 constexpr int kNT64 = 512;   // F64
@@ -746,6 +754,11 @@ void launch_fused(hipStream_t s, const FusedArgs &A, int f32k)
 void launch_controller(hipStream_t s, const CtlArgs &A)
 {
     hipLaunchKernelGGL(k_controller, dim3(1), dim3(1024), 0, s, A);
+}
+
+void launch_controller_batch(hipStream_t s, const CtlArgs *args, int n)
+{
+    hipLaunchKernelGGL(k_controller_batch, dim3(n), dim3(1024), 0, s, args);
 }
 
 }  // namespace gmd

@@ -61,6 +61,7 @@ struct gm_ctx {
     int TX[7] = {}, TY[7] = {}, tiles_x[7] = {}, tiles_y[7] = {};
     int CZ[7] = {}, nchunks[7] = {}, nblocks[7] = {};
     int ncu = 256;
+    int rect_cap = 0;             // > 0: rects at most this large (ensemble members: 512)
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -149,7 +150,8 @@ int stage_blocks_per_cu(const gm_ctx *c, int k)
 int stage_max_rect(const gm_ctx *c, int k)
 {
     if (c->prec != GM_PREC_F32K) return gmd::fused_max_points(0);
-    return gmd::max_rect32(k == 6 ? -1 : k, c->p.pressure);
+    const int m = gmd::max_rect32(k == 6 ? -1 : k, c->p.pressure);
+    return c->rect_cap > 0 ? std::min(m, c->rect_cap) : m;
 }
 
 void set_tile_k(gm_ctx *c, int k, int tx, int ty, int cz)
@@ -934,5 +936,206 @@ extern "C" int gm_write_device_info(const char *dir, int active, int precision_b
         fprintf(fd, "\n");
     }
     fclose(fd);
+    return GM_OK;
+}
+
+
+// ================================================================ ensemble
+// Many independent relaxations advanced together: every stage is one launch
+// over the workgroups of all members (each member keeps its own tiles,
+// buffers, dt and control block), and one controller launch applies each
+// member's accept/reject, dt and stop rules.  Members that are done return
+// at once from every later kernel of the batch (their convergence mask).
+struct gm_batch {
+    std::vector<gm_ctx *> inst;
+    hipStream_t stream = nullptr;
+    int press = 0;
+    gmd::FusedArgs *args[6] = {};   // per stage: one argument block per member
+    int *start[6] = {};             // per stage: first workgroup of each member, then the total
+    int total[6] = {};
+    gmd::CtlArgs *cargs = nullptr;
+    std::vector<gmd::Ctl *> ctlh;   // pinned host mirrors (the members' own)
+    hipGraphExec_t gexec = nullptr;
+    int gbatch = 0;
+};
+
+namespace {
+
+void batch_free_dev(gm_batch *b)
+{
+    for (int s = 0; s < 6; ++s) { dfree(b->args[s]); dfree(b->start[s]); }
+    dfree(b->cargs);
+}
+
+// Launch arguments from the members' current tiles and buffers.
+int batch_prepare(gm_batch *b)
+{
+    const int n = (int)b->inst.size();
+    batch_free_dev(b);
+    std::vector<gmd::FusedArgs> A(n);
+    std::vector<int> st(n + 1);
+    std::vector<gmd::CtlArgs> C(n);
+    int r;
+    for (int s = 0; s < 6; ++s) {
+        st[0] = 0;
+        for (int i = 0; i < n; ++i) {
+            A[i] = fused_args(b->inst[i], s, 0, s == 0 ? 1 : 0, 1);
+            st[i + 1] = st[i] + A[i].nblocks;
+        }
+        b->total[s] = st[n];
+        if ((r = dalloc(&b->args[s], (size_t)n)) || (r = dalloc(&b->start[s], (size_t)n + 1))) return r;
+        HIPCHK(hipMemcpy(b->args[s], A.data(), n * sizeof(gmd::FusedArgs), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->start[s], st.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    }
+    for (int i = 0; i < n; ++i) {
+        gm_ctx *c = b->inst[i];
+        gmd::CtlArgs &K = C[i];
+        K = gmd::CtlArgs{};
+        K.ctl = c->ctl;
+        K.part_delta = c->part_delta;
+        K.part_diag = c->part_diag;
+        K.nb_delta = c->nblocks[5];
+        K.nb_diag = c->nblocks[0];
+        K.diag = 1;
+        K.nbuf = 2;
+        K.maxError = c->p.maxError;
+        K.dtMin = c->p.dtMin;
+        K.cell = c->p.dx * c->p.dy * c->p.dz;
+        K.log = c->log;
+        K.log_cap = c->log_cap;
+    }
+    if ((r = dalloc(&b->cargs, (size_t)n))) return r;
+    HIPCHK(hipMemcpy(b->cargs, C.data(), n * sizeof(gmd::CtlArgs), hipMemcpyHostToDevice));
+    return GM_OK;
+}
+
+void batch_enqueue_step(gm_batch *b)
+{
+    const int n = (int)b->inst.size();
+    for (int s = 0; s < 6; ++s) gmd::launch_fused32_batch(b->stream, s, b->press, b->args[s], b->start[s], n, b->total[s]);
+    gmd::launch_controller_batch(b->stream, b->cargs, n);
+}
+
+}  // namespace
+
+extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm_batch **out)
+{
+    if (!params || !out || n < 1) return fail(GM_ERR_ARG, "gm_batch_create: bad arguments");
+    if (precision != GM_PREC_F32K) return fail(GM_ERR_UNSUPPORTED, "ensembles run the FP32 stage kernel (GM_PREC_F32K)");
+    for (int i = 0; i < n; ++i) {
+        if (params[i].pressure != params[0].pressure)
+            return fail(GM_ERR_ARG, "ensemble members must agree on pressure (one kernel variant per launch)");
+        if (any_peri(params[i])) return fail(GM_ERR_UNSUPPORTED, "ensemble members: periodic boundaries run the per-kernel path");
+    }
+    gm_batch *b = new gm_batch();
+    auto bail = [&](int code) {
+        gm_batch_destroy(b);
+        return code;
+    };
+    int r;
+    for (int i = 0; i < n; ++i) {
+        gm_ctx *c = nullptr;
+        if ((r = gm_create(&params[i], precision, &c))) return bail(r);
+        b->inst.push_back(c);
+        c->rect_cap = 512;
+        // one round over the CUs for the whole ensemble
+        auto_tiles(c, std::max(1, (c->ncu + n - 1) / n));
+        if ((r = ensure_partials(c))) return bail(r);
+        b->ctlh.push_back(c->ctlh);
+    }
+    b->press = params[0].pressure ? 1 : 0;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(GM_ERR_HIP, "stream"));
+    *out = b;
+    return GM_OK;
+}
+
+extern "C" int gm_batch_destroy(gm_batch *b)
+{
+    if (!b) return GM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    batch_free_dev(b);
+    for (gm_ctx *c : b->inst) gm_destroy(c);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return GM_OK;
+}
+
+extern "C" int gm_batch_context(gm_batch *b, int i, gm_ctx **ctx)
+{
+    if (!b || !ctx || i < 0 || i >= (int)b->inst.size()) return fail(GM_ERR_ARG, "gm_batch_context: bad member index");
+    *ctx = b->inst[i];
+    return GM_OK;
+}
+
+extern "C" int gm_batch_run(gm_batch *b, const gm_run_opts *o, gm_run_stats *st)
+{
+    if (!b || !o) return fail(GM_ERR_ARG, "gm_batch_run: null pointer");
+    const int n = (int)b->inst.size();
+    int r;
+    for (gm_ctx *c : b->inst) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if ((r = sync_ctl(c))) return r;
+        gmd::Ctl &h = *c->ctlh;
+        h.done = 0;
+        h.converged = 0;
+        h.stop_reason = 0;
+        h.nan_seen = 0;
+        h.diag_valid = 0;
+        h.stop_ratio = o->stop_ratio;
+        long long ncyc = o->n_cycle > 0 ? o->n_cycle : c->p.nCycle;
+        h.n_cycle = h.n_accept + ncyc;
+        h.max_attempt = h.n_attempt + (o->max_attempts > 0 ? o->max_attempts : (1LL << 40));
+        if ((r = push_ctl(c))) return r;
+    }
+    if ((r = batch_prepare(b))) return r;
+    const int batch = std::max(1, o->batch);
+    auto t0 = std::chrono::steady_clock::now();
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (o->use_graph) {
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < batch; ++i) batch_enqueue_step(b);
+        HIPCHK(hipStreamEndCapture(b->stream, &gr));
+        HIPCHK(hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        b->gbatch = batch;
+    }
+    while (true) {
+        if (b->gexec)
+            HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        else
+            for (int i = 0; i < batch; ++i) batch_enqueue_step(b);
+        HIPCHK(hipGetLastError());
+        for (int i = 0; i < n; ++i)
+            HIPCHK(hipMemcpyAsync(b->ctlh[i], b->inst[i]->ctl, sizeof(gmd::Ctl), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        bool all = true;
+        for (int i = 0; i < n; ++i) all = all && b->ctlh[i]->done;
+        if (all) break;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    bool nan = false;
+    for (int i = 0; i < n; ++i) {
+        gm_ctx *c = b->inst[i];
+        const gmd::Ctl &h = *c->ctlh;
+        c->cur = h.cur;
+        nan = nan || h.nan_seen;
+        if (st) {
+            gm_run_stats &s = st[i];
+            s.n_attempt = h.n_attempt;
+            s.n_accept = h.n_accept;
+            s.t = h.t;
+            s.dt = h.dt;
+            s.ratio = h.diag_valid ? h.ratio : -1;
+            s.ratio_pt = h.diag_valid ? h.ratio_pt : -1;
+            s.energy = h.diag_valid ? h.energy : -1;
+            s.converged = h.converged;
+            s.stopped = h.done && !h.converged;
+            s.nan_seen = h.nan_seen;
+            s.seconds = std::chrono::duration<double>(t1 - t0).count();
+        }
+    }
+    if (nan) return fail(GM_ERR_NAN, "NaN in the error estimate of an ensemble member");
     return GM_OK;
 }

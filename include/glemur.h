@@ -226,6 +226,21 @@ typedef struct gm_step_record {
 /* Copy the last n records (n <= capacity) into out; returns the count. */
 int gm_step_log(gm_ctx *ctx, gm_step_record *out, int n);
 
+/* ---- ensembles (no reference counterpart) ----------------------------- */
+/* n independent relaxations (sizes and parameters may differ; pressure must
+ * agree) advanced together in FP32 with compensated positions: each stage is
+ * one launch over the workgroups of all members, one controller launch
+ * applies every member's accept/reject, dt and stop rules, and finished
+ * members drop out of every later launch.  Members are ordinary contexts
+ * (gm_batch_context) for upload, download, step logs and tiles; they belong
+ * to the ensemble and are destroyed with it. */
+typedef struct gm_batch gm_batch;
+int gm_batch_create(const gm_params *params, int n, int precision, gm_batch **out);
+int gm_batch_destroy(gm_batch *b);
+int gm_batch_context(gm_batch *b, int i, gm_ctx **ctx);
+/* Like gm_run for every member at once; stats has n entries (or NULL). */
+int gm_batch_run(gm_batch *b, const gm_run_opts *opts, gm_run_stats *stats);
+
 /* ---- kernel timing (HIP events on the context stream) ---------------- */
 /* Enable/disable event timing of the fused stage kernels (resets the
  * totals); gm_timing_read returns the summed milliseconds and launch count
