@@ -64,9 +64,12 @@ def traffic_for(traffic, prec, tile):
     return t[1] if t and t[0] == tuple(tile) else None
 
 
-def measure(g, p, B0, xb, precision, steps, warmup, dt):
+def measure(g, p, B0, xb, precision, steps, warmup, dt, autotune=False):
     ctx = g.Context(p, precision)
     ctx.upload(B0, xb, 0.0, dt)
+    tuned = None
+    if autotune:
+        tuned = {"ms": ctx.autotune(reps=5), "tiles": [ctx.stage_tile(s) for s in range(6)]}
     # warm-up attempts (not timed)
     ctx.run(max_attempts=warmup, n_cycle=10 ** 9, batch=max(1, warmup))
     # per-launch HIP-event timing of the stage kernels over `steps` attempts
@@ -94,6 +97,7 @@ def measure(g, p, B0, xb, precision, steps, warmup, dt):
         "ms_controller": ms[6] / max(1, cnt[6]),
         "launches": int(sum(cnt[:6])),
         "tile": tuple(tile),
+        "autotune": tuned,
     }
 
 
@@ -168,6 +172,8 @@ def main():
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--relax-seconds", type=float, default=10.0)
     ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0: skip)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick each stage kernel's tile by timing candidates before measuring")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -191,7 +197,7 @@ def main():
     # the first tiny step
     dt0 = 0.02
 
-    f32 = measure(g, p, B0, xb, g.PREC_F32K, args.steps, args.warmup, dt0)
+    f32 = measure(g, p, B0, xb, g.PREC_F32K, args.steps, args.warmup, dt0, args.autotune)
     value = n_pts * 6 * f32["attempts"] / f32["seconds"]
     bpp = g.algorithmic_bytes(g.PREC_F32K)
     bytes_launch = bpp * n_pts
@@ -228,6 +234,9 @@ def main():
                      "controller_ms": f32["ms_controller"],
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic_f32 else None},
     }
+    if f32["autotune"]:
+        out["autotune"] = {"stage_tiles_tx_ty_cz_workgroups": f32["autotune"]["tiles"],
+                           "stage_ms": f32["autotune"]["ms"]}
     if copy_gbs is not None:
         out["copy_probe_GBs"] = copy_gbs
     if not args.no_f64:

@@ -157,6 +157,8 @@ _SIGS = {
     "gm_write_ts": (C.c_int, [C.c_char_p, _P(Params), _P(Reduction), C.c_double, C.c_double, C.c_int,
                                C.c_double, C.c_int, C.c_int]),
     "gm_write_device_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int]),
+    "gm_get_stage_tile": (C.c_int, [_CTX, C.c_int, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
+    "gm_autotune": (C.c_int, [_CTX, C.c_int, _D]),
     "gm_batch_create": (C.c_int, [_P(Params), C.c_int, C.c_int, _P(_CTX)]),
     "gm_batch_destroy": (C.c_int, [_CTX]),
     "gm_batch_context": (C.c_int, [_CTX, C.c_int, _P(_CTX)]),
@@ -348,6 +350,19 @@ class Context:
 
     def set_tile(self, tx, ty, cz):
         _check(_lib.gm_set_tile(self._h, tx, ty, cz))
+
+    def stage_tile(self, stage):
+        """(tx, ty, cz, workgroups) of one stage kernel (-1: fields pass)."""
+        v = [C.c_int() for _ in range(4)]
+        _check(_lib.gm_get_stage_tile(self._h, stage, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune(self, reps=3):
+        """Pick each stage kernel's tile by timing candidates; returns the
+        winners' mean launch times (ms) for stages 0..5."""
+        ms = (C.c_double * 6)()
+        _check(_lib.gm_autotune(self._h, int(reps), ms))
+        return list(ms)
 
     def tile(self):
         v = [C.c_int() for _ in range(4)]

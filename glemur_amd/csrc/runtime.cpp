@@ -940,6 +940,96 @@ extern "C" int gm_write_device_info(const char *dir, int active, int precision_b
 }
 
 
+
+// ================================================================ autotune
+namespace {
+
+// Fewest loaded points per output point among the rects with
+// lo < (TX+4)(TY+4) <= hi.
+bool best_rect(const gm_ctx *c, int lo, int hi, int &bx, int &by)
+{
+    const int nx = c->p.nx, ny = c->p.ny;
+    double best = 1e30;
+    for (int ty = 2; ty <= 96; ++ty)
+        for (int txm = 4; (txm + 4) * (ty + 4) <= hi; ++txm) {
+            int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
+            int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
+            int rect = (tx + 4) * (tyy + 4);
+            if (rect > hi || rect <= lo) continue;
+            double cost = (double)tlx * tly * rect / ((double)nx * ny);
+            if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
+        }
+    return best < 1e30;
+}
+
+}  // namespace
+
+// Per stage kernel: time `reps` launches of each candidate tile on the
+// current state and keep the fastest.  The stage kernels alone write only
+// the slope buffers and the candidate position buffer, so the accepted
+// state, t and dt are unchanged.
+extern "C" int gm_autotune(gm_ctx *c, int reps, double ms_best[6])
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = check_fused(c)) return r;
+    if (c->timing) return fail(GM_ERR_ARG, "gm_autotune: disable kernel timing first");
+    reps = std::max(1, reps);
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    const int f32k = c->prec == GM_PREC_F32K;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int bounds[5] = {0, 256, 384, 512, 1024};
+    int rc = GM_OK;
+    for (int k = 0; k < 7 && rc == GM_OK; ++k) {
+        const int stage = k == 6 ? -1 : k;
+        const int cap = stage_max_rect(c, k);
+        int bTX = c->TX[k], bTY = c->TY[k], bCZ = c->CZ[k];
+        double best = 1e30;
+        for (int w = 0; w < 4 && rc == GM_OK; ++w) {
+            if (bounds[w] >= cap) break;
+            int tx, ty;
+            if (!best_rect(c, bounds[w], std::min(bounds[w + 1], cap), tx, ty)) continue;
+            for (int rounds = 1; rounds <= 2; ++rounds) {
+                set_tile_k(c, k, tx, ty, c->p.nz);
+                const int cols = c->tiles_x[k] * c->tiles_y[k];
+                int nch = std::max(1, c->ncu * rounds * stage_blocks_per_cu(c, k) / cols);
+                nch = std::min(nch, std::max(1, c->p.nz / 8));
+                set_tile_k(c, k, tx, ty, (c->p.nz + nch - 1) / nch);
+                if ((rc = ensure_partials(c))) break;
+                gmd::FusedArgs A = fused_args(c, stage, 0, stage == 0 ? 1 : 0, 0);
+                gmd::launch_fused(c->stream, A, f32k);   // warm (and first-launch setup)
+                if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(GM_ERR_HIP, "event"); break; }
+                for (int i = 0; i < reps; ++i) gmd::launch_fused(c->stream, A, f32k);
+                if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = fail(GM_ERR_HIP, "event");
+                    break;
+                }
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms / reps < best) { best = ms / reps; bTX = tx; bTY = ty; bCZ = c->CZ[k]; }
+            }
+        }
+        set_tile_k(c, k, bTX, bTY, bCZ);
+        if (ms_best && k < 6) ms_best[k] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return ensure_partials(c);
+}
+
+extern "C" int gm_get_stage_tile(gm_ctx *c, int stage, int *tx, int *ty, int *cz, int *nblocks)
+{
+    if (!c || stage < -1 || stage > 5) return fail(GM_ERR_ARG, "gm_get_stage_tile: bad arguments");
+    const int k = stage < 0 ? 6 : stage;
+    if (tx) *tx = c->TX[k];
+    if (ty) *ty = c->TY[k];
+    if (cz) *cz = c->CZ[k];
+    if (nblocks) *nblocks = c->nblocks[k];
+    return GM_OK;
+}
+
 // ================================================================ ensemble
 // Many independent relaxations advanced together: every stage is one launch
 // over the workgroups of all members (each member keeps its own tiles,

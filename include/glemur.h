@@ -258,6 +258,16 @@ int gm_copy_probe(int64_t bytes, int reps, double *ms_mean);
 /* tx or ty <= 0 picks the tile automatically for about `cz` workgroups. */
 int gm_set_tile(gm_ctx *ctx, int tx, int ty, int cz);
 int gm_get_tile(gm_ctx *ctx, int *tx, int *ty, int *cz, int *nblocks);
+/* Tile of one stage kernel (stage 0..5, -1 the fields-only pass); each stage
+ * kernel has its own (gm_set_tile sets them all alike). */
+int gm_get_stage_tile(gm_ctx *ctx, int stage, int *tx, int *ty, int *cz, int *nblocks);
+/* Tile autotuner (no reference counterpart): for every stage kernel, times
+ * `reps` launches of each candidate rect (the fewest-loads rect for each
+ * workgroup size the kernel is built for, with one or two rounds of
+ * workgroups over the CUs) on the current state and keeps the fastest;
+ * ms_best (6 entries, or NULL) receives the winners' mean times.  The
+ * accepted state, t and dt are unchanged. */
+int gm_autotune(gm_ctx *ctx, int reps, double ms_best[6]);
 
 /* ---- files (src/io.cu) ------------------------------------------------ */
 /* writeB0 (src/io.cu:167-205): data/B0.vtk in dir. */
