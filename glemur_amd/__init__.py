@@ -110,6 +110,16 @@ class StepRecord(C.Structure):
                 ("accepted", C.c_int32), ("nan_seen", C.c_int32)]
 
 
+class WarmInfo(C.Structure):
+    _fields_ = [("used", C.c_int), ("mesh_ok", C.c_int), ("settle_attempts", C.c_int64),
+                ("ratio_cold", C.c_double), ("ratio_applied", C.c_double), ("ratio_warm", C.c_double),
+                ("energy_cold", C.c_double), ("energy_applied", C.c_double), ("energy_warm", C.c_double),
+                ("min_det", C.c_double), ("ms", C.c_double)]
+
+
+WARM_FORCE = 1
+
+
 _P = C.POINTER
 _D = _P(C.c_double)
 _CTX = C.c_void_p
@@ -163,6 +173,19 @@ _SIGS = {
     "gm_batch_destroy": (C.c_int, [_CTX]),
     "gm_batch_context": (C.c_int, [_CTX, C.c_int, _P(_CTX)]),
     "gm_batch_run": (C.c_int, [_CTX, _P(RunOpts), _P(RunStats)]),
+    "gm_unet_num_params": (C.c_int64, []),
+    "gm_unet_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _P(_CTX)]),
+    "gm_unet_destroy": (C.c_int, [_CTX]),
+    "gm_unet_init_params": (C.c_int, [_CTX, C.c_uint64]),
+    "gm_unet_set_params": (C.c_int, [_CTX, _D, C.c_int64]),
+    "gm_unet_get_params": (C.c_int, [_CTX, _D, C.c_int64]),
+    "gm_unet_forward": (C.c_int, [_CTX, _D, _D]),
+    "gm_unet_loss_grad": (C.c_int, [_CTX, _D, _D, _D, _D]),
+    "gm_unet_add_sample": (C.c_int, [_CTX, _CTX]),
+    "gm_unet_num_samples": (C.c_int, [_CTX]),
+    "gm_unet_clear_samples": (C.c_int, [_CTX]),
+    "gm_unet_train": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_double, C.c_uint64, _D]),
+    "gm_warm_start": (C.c_int, [_CTX, _CTX, C.c_int, C.c_int, _P(WarmInfo)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -441,3 +464,82 @@ def read_state(dirpath, p, precision_bits=64):
 def write_ts(dirpath, p, red, t, dt, it, maxDelta, echo=False, jxb_jb_col=True):
     _check(_lib.gm_write_ts(str(dirpath).encode(), C.byref(p), C.byref(red) if red is not None else None,
                             t, dt, it, maxDelta, int(echo), int(jxb_jb_col)))
+
+
+class UNet:
+    """Warm-start U-Net on an nx x ny x nz grid (each 4m+1); `capacity`
+    training samples are held in device memory."""
+
+    def __init__(self, nx, ny, nz, capacity=0):
+        self.shape = (nz, ny, nx)
+        self._h = _CTX()
+        _check(_lib.gm_unet_create(nx, ny, nz, capacity, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            _lib.gm_unet_destroy(self._h)
+            self._h = _CTX()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def num_params():
+        return int(_lib.gm_unet_num_params())
+
+    def init_params(self, seed):
+        _check(_lib.gm_unet_init_params(self._h, C.c_uint64(seed)))
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        _check(_lib.gm_unet_set_params(self._h, _dptr(params), params.size))
+
+    def get_params(self):
+        out = np.empty(self.num_params())
+        _check(_lib.gm_unet_get_params(self._h, _dptr(out), out.size))
+        return out
+
+    def forward(self, feat):
+        """feat [nz][ny][nx][8] -> prediction [nz][ny][nx][3]."""
+        feat = np.ascontiguousarray(feat, dtype=np.float64)
+        assert feat.shape == self.shape + (8,)
+        y = np.empty(self.shape + (3,))
+        _check(_lib.gm_unet_forward(self._h, _dptr(feat), _dptr(y)))
+        return y
+
+    def loss_grad(self, feat, target):
+        feat = np.ascontiguousarray(feat, dtype=np.float64)
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        assert feat.shape == self.shape + (8,) and target.shape == self.shape + (3,)
+        loss = C.c_double()
+        grad = np.empty(self.num_params() - 1)
+        _check(_lib.gm_unet_loss_grad(self._h, _dptr(feat), _dptr(target), C.byref(loss), _dptr(grad)))
+        return loss.value, grad
+
+    def add_sample(self, ctx):
+        _check(_lib.gm_unet_add_sample(self._h, ctx._h))
+
+    def num_samples(self):
+        return _lib.gm_unet_num_samples(self._h)
+
+    def clear_samples(self):
+        _check(_lib.gm_unet_clear_samples(self._h))
+
+    def train(self, epochs, batch=8, lr=1e-3, seed=0):
+        loss = C.c_double()
+        _check(_lib.gm_unet_train(self._h, epochs, batch, lr, C.c_uint64(seed), C.byref(loss)))
+        return loss.value
+
+    def warm_start(self, ctx, settle=0, force=False):
+        info = WarmInfo()
+        _check(_lib.gm_warm_start(ctx._h, self._h, int(settle), WARM_FORCE if force else 0, C.byref(info)))
+        return info

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "gm_ctx.hpp"
 #include "gm_device.hpp"
 #include "gm_fused.hpp"
 #include "gm_internal.hpp"
@@ -24,55 +25,6 @@ using gm::fail;
             return fail(GM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
-struct gm_ctx {
-    gm_params p{};
-    int prec = GM_PREC_F64;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    gmd::Geo geo{};
-    long long P = 0, N = 0;
-    // F64
-    double *X64[3] = {nullptr, nullptr, nullptr};
-    double *B0_64 = nullptr, *K64 = nullptr, *B64 = nullptr, *J64 = nullptr, *G64 = nullptr, *Xtmp64 = nullptr;
-    // F32K
-    float *Xh[2] = {nullptr, nullptr}, *Xl[2] = {nullptr, nullptr};
-    float *B0_32 = nullptr, *K32 = nullptr, *B32 = nullptr, *J32 = nullptr, *G32 = nullptr;
-    double *det = nullptr;
-    double *epsx = nullptr, *epsy = nullptr, *epsz = nullptr;
-    gmd::Ctl *ctl = nullptr;      // device
-    gmd::Ctl *ctlh = nullptr;     // pinned host mirror
-    double *part_delta = nullptr, *part_diag = nullptr;
-    int part_cap = 0;
-    gmd::Tableau *tab_dev = nullptr;
-    gmd::TableauF tabf{};
-    gmd::GhostPt *ghost = nullptr;
-    long long ghost_off[6] = {0, 0, 0, 0, 0, 0};
-    gmd::StepRecord *log = nullptr;
-    int log_cap = 4096;
-    double *red_out = nullptr, *red_host = nullptr;   // 4 doubles each
-    double *diag_scratch = nullptr;
-    int diag_blocks = 2048;
-    int *nanflag = nullptr;
-    int cur = 0;                  // host copy of ctl->cur (valid after syncs)
-    // tiles
-    // tiles per stage kernel (index 0..5 the stages, 6 the fields-only
-    // pass): the stages exchange data through HBM only, so each takes the
-    // rect and z chunking that suit its own workgroup size and occupancy
-    int TX[7] = {}, TY[7] = {}, tiles_x[7] = {}, tiles_y[7] = {};
-    int CZ[7] = {}, nchunks[7] = {}, nblocks[7] = {};
-    int ncu = 256;
-    int rect_cap = 0;             // > 0: rects at most this large (ensemble members: 512)
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> ev_kind;
-    int ev_used = 0;
-    double tms[GM_TIMING_KINDS] = {};        // per stage 0..5, controller
-    long long tcount[GM_TIMING_KINDS] = {};
-    // graph of one batch
-    hipGraphExec_t gexec = nullptr;
-    int gbatch = 0;
-};
 
 namespace {
 

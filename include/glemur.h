@@ -241,6 +241,58 @@ int gm_batch_context(gm_batch *b, int i, gm_ctx **ctx);
 /* Like gm_run for every member at once; stats has n entries (or NULL). */
 int gm_batch_run(gm_batch *b, const gm_run_opts *opts, gm_run_stats *stats);
 
+/* ---- learned warm start (no reference counterpart) -------------------- */
+/* A 3-level U-Net (3x3x3 convolutions with 32/64/128 channels, 3x3x3
+ * average pooling with stride 2 down, trilinear x2 up, skip connections;
+ * bfloat16 activations, FP32 accumulation on the bf16 matrix cores) maps
+ * B0 of a braid on an n1 x n2 x n3 grid (each 4m+1) to the displacement of
+ * its relaxed positions.  Input features per point: Bx, By, Bz - 1 and the
+ * three coordinates scaled to [-1, 1].  Parameters are packed layer by
+ * layer, W[co][t][ci] (tap t = (dx+1) + 3(dy+1) + 9(dz+1)) then the biases,
+ * and last the output scale (prediction = scale * head output). */
+typedef struct gm_unet gm_unet;
+typedef struct gm_warm_info {
+    int     used;          /* 1: warm state kept; 0: fell back to the cold state */
+    int     mesh_ok;       /* min det > 0 and no NaN in the warm state */
+    int64_t settle_attempts;           /* attempts run on the warm state before the check */
+    double  ratio_cold, ratio_applied, ratio_warm;      /* max|JxB| / max(|J||B|): cold, right
+                                          after the prediction, after the settle attempts */
+    double  energy_cold, energy_applied, energy_warm;   /* B2 at the same three points */
+    double  min_det;       /* of the warm state */
+    double  ms;            /* wall time of the whole call */
+} gm_warm_info;
+#define GM_WARM_FORCE 1    /* keep the warm state whatever the checks say */
+int64_t gm_unet_num_params(void);
+/* capacity: training samples held in device memory (P * 28 bytes each). */
+int gm_unet_create(int nx, int ny, int nz, int capacity, gm_unet **out);
+int gm_unet_destroy(gm_unet *u);
+/* He-normal weights from a splitmix64 stream; the output scale is set from
+ * the data by the first gm_unet_train. */
+int gm_unet_init_params(gm_unet *u, uint64_t seed);
+int gm_unet_set_params(gm_unet *u, const double *params, int64_t n);   /* resets the optimiser */
+int gm_unet_get_params(gm_unet *u, double *params, int64_t n);
+/* Host features [P][8] -> prediction [P][3] (scale applied). */
+int gm_unet_forward(gm_unet *u, const double *feat, double *y);
+/* One sample: loss = mean over P and 3 of (head - target / scale)^2 and its
+ * gradient in the packed parameters (n - 1 entries, no scale), or NULL. */
+int gm_unet_loss_grad(gm_unet *u, const double *feat, const double *target, double *loss, double *grad);
+/* Add the context's B0 features and its positions minus the uniform grid
+ * as one training sample (the context is on the network's grid). */
+int gm_unet_add_sample(gm_unet *u, gm_ctx *ctx);
+int gm_unet_num_samples(gm_unet *u);
+int gm_unet_clear_samples(gm_unet *u);
+/* Adam (lr, 0.9, 0.999, 1e-8) over shuffled mini-batches of the resident
+ * samples; loss = mean sample loss of the last epoch. */
+int gm_unet_train(gm_unet *u, int epochs, int batch, double lr, uint64_t seed, double *loss);
+/* Warm start of a context on the network's grid or its 2x refinement
+ * (2n-1 points per axis): positions += trilinear x2 up-sampling of the
+ * prediction, then `settle` attempted steps (gm_run) so that the
+ * prediction's small-scale errors relax.  The warm state is kept when the
+ * mesh stays valid and both max|JxB|/max(|J||B|) and the energy are then
+ * below the cold state's (or flags has GM_WARM_FORCE); otherwise the cold
+ * positions, time, dt and counters are restored (info->used = 0). */
+int gm_warm_start(gm_ctx *ctx, gm_unet *u, int settle, int flags, gm_warm_info *info);
+
 /* ---- kernel timing (HIP events on the context stream) ---------------- */
 /* Enable/disable event timing of the fused stage kernels (resets the
  * totals); gm_timing_read returns the summed milliseconds and launch count

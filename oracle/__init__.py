@@ -193,3 +193,145 @@ def set_threads(n):
 
 def num_threads():
     return _lib.or_num_threads()
+
+
+# ---------------------------------------------------------------- U-Net
+_I = C.c_int
+for _n, _a in {
+    "ou_conv3d": [_D, _I, _I, _I, _I, _I, _D, _D, _I, _I, _D, _I],
+    "ou_conv3d_bwd_data": [_D, _I, _I, _I, _I, _I, _D, _I, _D, _I],
+    "ou_conv3d_bwd_weight": [_D, _I, _D, _I, _I, _I, _I, _I, _I, _D, _D],
+    "ou_pool": [_D, _I, _I, _I, _I, _I, _D, _I],
+    "ou_pool_bwd": [_D, _I, _I, _I, _I, _I, _D, _I],
+    "ou_up": [_D, _I, _I, _I, _I, _I, _D, _I],
+    "ou_up_bwd": [_D, _I, _I, _I, _I, _I, _D, _I],
+    "ou_unet_features": [_D, _I, _I, _I, _I, _I, _I, _I, _D],
+    "ou_unet_loss": [_D, _D, _D, _I, _I, _I, _I, _D, _D],
+}.items():
+    getattr(_lib, _n).argtypes = _a
+    getattr(_lib, _n).restype = None
+_lib.ou_unet_loss.restype = C.c_double
+_lib.ou_bf16.argtypes = [C.c_double]
+_lib.ou_bf16.restype = C.c_double
+_lib.ou_unet_nparams.restype = C.c_long
+
+UNET_LAYERS = [(8, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128),
+               (192, 64), (64, 64), (96, 32), (32, 32), (32, 3)]
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _opt(a):
+    return None if a is None else dp(a)
+
+
+def bf16(a):
+    """Round to bfloat16 (nearest even), as float64."""
+    a = np.asarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    fin = (u & 0x7F800000) != 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    out = np.where(fin, r, u.astype(np.uint32)).view(np.float32)
+    return out.astype(np.float64)
+
+
+def conv3d(x, w, b, relu=False):
+    """x [nz][ny][nx][cin], w [cout][27][cin], b [cout] or None -> [nz][ny][nx][cout]."""
+    x, w = _c(x), _c(w)
+    nz, ny, nx, cin = x.shape
+    cout = w.shape[0]
+    out = np.zeros((nz, ny, nx, cout))
+    bb = None if b is None else _c(b)
+    _lib.ou_conv3d(dp(x), cin, nx, ny, nz, cin, dp(w), _opt(bb), cout, int(relu), dp(out), cout)
+    return out
+
+
+def conv3d_bwd_data(gy, w):
+    gy, w = _c(gy), _c(w)
+    nz, ny, nx, cout = gy.shape
+    cin = w.shape[2]
+    gx = np.zeros((nz, ny, nx, cin))
+    _lib.ou_conv3d_bwd_data(dp(gy), cout, nx, ny, nz, cout, dp(w), cin, dp(gx), cin)
+    return gx
+
+
+def conv3d_bwd_weight(x, gy):
+    x, gy = _c(x), _c(gy)
+    nz, ny, nx, cin = x.shape
+    cout = gy.shape[3]
+    gw, gb = np.zeros((cout, 27, cin)), np.zeros(cout)
+    _lib.ou_conv3d_bwd_weight(dp(x), cin, dp(gy), cout, nx, ny, nz, cin, cout, dp(gw), dp(gb))
+    return gw, gb
+
+
+def pool(x):
+    x = _c(x)
+    nz, ny, nx, c = x.shape
+    out = np.zeros(((nz + 1) // 2, (ny + 1) // 2, (nx + 1) // 2, c))
+    _lib.ou_pool(dp(x), c, nx, ny, nz, c, dp(out), c)
+    return out
+
+
+def pool_bwd(g, fine_shape):
+    g = _c(g)
+    nz, ny, nx = fine_shape
+    c = g.shape[3]
+    out = np.zeros((nz, ny, nx, c))
+    _lib.ou_pool_bwd(dp(g), c, nx, ny, nz, c, dp(out), c)
+    return out
+
+
+def up(x):
+    x = _c(x)
+    nz, ny, nx, c = x.shape
+    out = np.zeros((2 * nz - 1, 2 * ny - 1, 2 * nx - 1, c))
+    _lib.ou_up(dp(x), c, nx, ny, nz, c, dp(out), c)
+    return out
+
+
+def up_bwd(g):
+    g = _c(g)
+    fz, fy, fx, c = g.shape
+    nz, ny, nx = (fz + 1) // 2, (fy + 1) // 2, (fx + 1) // 2
+    out = np.zeros((nz, ny, nx, c))
+    _lib.ou_up_bwd(dp(g), c, nx, ny, nz, c, dp(out), c)
+    return out
+
+
+def unet_nparams():
+    return int(_lib.ou_unet_nparams())
+
+
+def unet_features(B0pad, fine, stride, coarse):
+    """B0pad: AoS padded B0 of the fine grid (fnx, fny, fnz); features of the
+    coarse grid (nx, ny, nz) sampled every `stride` fine points."""
+    B0pad = _c(B0pad)
+    nx, ny, nz = coarse
+    f = np.zeros((nz, ny, nx, 8))
+    _lib.ou_unet_features(dp(B0pad), fine[0], fine[1], fine[2], stride, nx, ny, nz, dp(f))
+    return f
+
+
+def unet_loss(params, feat, target, round_bf16=False, grad=True):
+    """(loss, y [nz][ny][nx][3], gradient or None) of the U-Net."""
+    params, feat, target = _c(params), _c(feat), _c(target)
+    nz, ny, nx = feat.shape[:3]
+    y = np.zeros((nz, ny, nx, 3))
+    g = np.zeros(params.size - 1) if grad else None
+    loss = _lib.ou_unet_loss(dp(params), dp(feat), dp(target), nx, ny, nz, int(round_bf16), dp(y), _opt(g))
+    return loss, y, g
+
+
+def unet_init(seed, scale=1.0):
+    """He-normal weights (std sqrt(2 / (27 cin))), zero biases, head std
+    scaled by 0.1; packed layer by layer, then the output scale."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l, (cin, cout) in enumerate(UNET_LAYERS):
+        std = np.sqrt(2.0 / (27 * cin)) * (0.1 if l == len(UNET_LAYERS) - 1 else 1.0)
+        parts.append(rng.standard_normal(cout * 27 * cin) * std)
+        parts.append(np.zeros(cout))
+    parts.append(np.array([scale]))
+    return np.concatenate(parts)

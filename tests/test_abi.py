@@ -51,6 +51,7 @@ STRUCTS = {
     "gm_run_opts": g.RunOpts,
     "gm_run_stats": g.RunStats,
     "gm_step_record": g.StepRecord,
+    "gm_warm_info": g.WarmInfo,
 }
 
 

@@ -1,0 +1,187 @@
+"""Warm-start U-Net on the GPU (bf16 matrix-core convolutions, FP32
+accumulation) against the oracle's U-Net with the same bfloat16 rounding
+points, and against the exact FP64 network; then training on a relaxation
+held in device memory and the warm start with its fallback.
+
+Tolerances (max-norm, relative to the largest entry of the reference):
+the GPU stores every activation and gradient in bfloat16 exactly where the
+oracle rounds, and accumulates in FP32 where the oracle uses FP64, so the
+two differ by roundings that fall on the other side of a bf16 tie:
+prediction 1e-2, loss 1e-3, each layer's weight gradient 3e-2.  Against
+the unrounded FP64 network: prediction 5e-2."""
+import numpy as np
+import pytest
+
+from conftest import small_braid
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g(require_gpu):
+    import glemur_amd
+    return glemur_amd
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def case(seed, shape):
+    import oracle as o
+    rng = np.random.default_rng(seed)
+    params = o.unet_init(seed, scale=0.5)
+    params[:-1] += 0.02 * rng.standard_normal(params.size - 1)
+    feat = o.bf16(rng.standard_normal(shape + (8,)))
+    target = rng.standard_normal(shape + (3,)) * 0.3
+    return params, feat, target
+
+
+SHAPE = (17, 13, 9)   # nz, ny, nx: each 4m+1, distinct
+
+
+def test_forward_matches_oracle(g):
+    import oracle as o
+    params, feat, target = case(11, SHAPE)
+    with g.UNet(SHAPE[2], SHAPE[1], SHAPE[0]) as u:
+        u.set_params(params)
+        assert np.array_equal(u.get_params(), params.astype(np.float32).astype(np.float64))
+        y = u.forward(feat)
+    _, y_r, _ = o.unet_loss(params, feat, target, round_bf16=True, grad=False)
+    _, y_x, _ = o.unet_loss(params, feat, target, round_bf16=False, grad=False)
+    e_r, e_x = relerr(y, y_r), relerr(y, y_x)
+    print(f"U-Net forward: vs bf16-rounded oracle {e_r:.2e}, vs FP64 {e_x:.2e}, max|y| {np.abs(y_r).max():.3f}")
+    assert e_r < 1e-2 and e_x < 5e-2
+
+
+def test_loss_and_gradient_match_oracle(g):
+    import oracle as o
+    params, feat, target = case(12, SHAPE)
+    with g.UNet(SHAPE[2], SHAPE[1], SHAPE[0]) as u:
+        u.set_params(params)
+        loss, grad = u.loss_grad(feat, target)
+        loss2, grad2 = u.loss_grad(feat, target)
+    assert loss == loss2 and np.array_equal(grad, grad2)   # deterministic reductions
+    l_r, _, g_r = o.unet_loss(params, feat, target, round_bf16=True)
+    assert abs(loss - l_r) < 1e-3 * l_r
+    off = 0
+    for l, (cin, cout) in enumerate(o.UNET_LAYERS):
+        n = cout * 27 * cin + cout
+        e = relerr(grad[off:off + n], g_r[off:off + n])
+        print(f"layer {l} ({cin}->{cout}): gradient error {e:.2e}")
+        assert e < 3e-2, l
+        off += n
+
+
+def relaxed_braid(g, n, steps):
+    """A braid on an (n, n, 2n-1) grid relaxed by `steps` attempts (F32K)."""
+    p = small_braid(n, n, 2 * n - 1)
+    p.nCycle = 10 ** 6
+    B0, xb = g.init_state(p)
+    c = g.Context(p, g.PREC_F32K)
+    c.upload(B0, xb, 0.0, 0.01)
+    c.run(max_attempts=steps, batch=50)
+    return p, B0, xb, c
+
+
+def test_train_and_warm_start(g):
+    """Overfit one relaxation, then warm-start a cold copy of it on the same
+    grid and on the 2x refined grid: the warm positions are the cold ones
+    plus the (up-sampled) prediction, and the fallback rules hold."""
+    import oracle as o
+    n = 17
+    p, B0, xb, c = relaxed_braid(g, n, 300)
+    with g.UNet(n, n, 2 * n - 1, capacity=2) as u:
+        u.init_params(5)
+        u.add_sample(c)
+        assert u.num_samples() == 1
+        l0 = u.train(1, batch=1, lr=1e-3)
+        l1 = u.train(200, batch=1, lr=1e-3, seed=1)
+        print(f"training loss {l0:.3e} -> {l1:.3e}")
+        assert l1 < 0.2 * l0
+        c.close()
+        # same grid (stride 1)
+        feat = o.unet_features(B0, (n, n, 2 * n - 1), 1, (n, n, 2 * n - 1))
+        y = u.forward(feat)
+        with g.Context(p, g.PREC_F32K) as cold:
+            cold.upload(B0, xb, 0.0, 0.01)
+            info = u.warm_start(cold, settle=0, force=True)
+            x = cold.download(g.FIELD_XB) + cold.download(g.FIELD_XB_LO)
+            assert cold.time() == (0.0, 0.01)
+        print("warm start (same grid, forced):", info.used, info.mesh_ok, info.ratio_cold, info.ratio_applied,
+              info.energy_cold, info.energy_applied, info.min_det, f"{info.ms:.1f} ms")
+        assert info.mesh_ok and info.used and info.energy_applied < info.energy_cold
+        sh = (2 * n + 1, n + 2, n + 2, 3)
+        d = (x.reshape(sh) - xb.reshape(sh))[1:-1, 1:-1, 1:-1]
+        assert np.max(np.abs(d - y)) < 1e-5 * max(1.0, np.abs(y).max())
+        assert np.array_equal(x.reshape(sh)[0], xb.reshape(sh)[0])   # ghosts untouched
+        # with settle attempts and the checks: either kept (and then better
+        # than the cold state) or the cold state is back exactly
+        with g.Context(p, g.PREC_F32K) as cold:
+            cold.upload(B0, xb, 0.0, 0.01)
+            info = u.warm_start(cold, settle=20)
+            print("warm start (same grid, settle 20):", info.used, info.settle_attempts, info.ratio_cold,
+                  info.ratio_applied, info.ratio_warm, info.energy_cold, info.energy_warm)
+            if info.used:
+                assert info.settle_attempts == 20 and cold.time()[0] > 0
+                assert info.ratio_warm < info.ratio_cold and info.energy_warm <= info.energy_cold
+            else:
+                assert cold.time() == (0.0, 0.01)
+                assert np.array_equal(cold.download(g.FIELD_XB), xb.astype(np.float32).astype(np.float64))
+        # 2x refined grid (stride 2): positions + trilinear up-sampling
+        q = small_braid(2 * n - 1, 2 * n - 1, 4 * n - 3)
+        B0f, xbf = g.init_state(q)
+        with g.Context(q, g.PREC_F64) as fine:
+            fine.upload(B0f, xbf, 0.0, 0.01)
+            info2 = u.warm_start(fine, force=True)
+            xf = fine.download(g.FIELD_XB)
+        print("warm start (2x grid, forced):", info2.used, info2.ratio_cold, info2.ratio_applied)
+        shf = (4 * n - 1, 2 * n + 1, 2 * n + 1, 3)
+        df = (xf.reshape(shf) - xbf.reshape(shf))[1:-1, 1:-1, 1:-1]
+        feat_f = o.unet_features(B0f, (2 * n - 1, 2 * n - 1, 4 * n - 3), 2, (n, n, 2 * n - 1))
+        assert np.array_equal(feat_f, feat)   # nested grids: the same coarse samples
+        assert np.max(np.abs(df - o.up(y))) < 1e-5 * max(1.0, np.abs(y).max())
+
+
+def test_fallback_restores_the_cold_state(g):
+    """A network whose prediction folds the mesh (huge output scale) and one
+    that predicts nothing (zero head) both leave the cold positions."""
+    import oracle as o
+    n = 9
+    p = small_braid(n, n, 2 * n - 1)
+    B0, xb = g.init_state(p)
+    params = o.unet_init(3, scale=1e3)
+    with g.UNet(n, n, 2 * n - 1) as u:
+        u.set_params(params)
+        for prec in (g.PREC_F64, g.PREC_F32K):
+            with g.Context(p, prec) as c:
+                c.upload(B0, xb, 0.0, 0.01)
+                info = u.warm_start(c, settle=5)
+                assert not info.used and not info.mesh_ok and info.settle_attempts == 0
+                assert c.time() == (0.0, 0.01)
+                assert np.array_equal(c.download(g.FIELD_XB), xb if prec == g.PREC_F64 else
+                                      xb.astype(np.float32).astype(np.float64))
+        offs = np.cumsum([0] + [co * 27 * ci + co for ci, co in o.UNET_LAYERS])
+        params[offs[-2]:offs[-1]] = 0.0
+        u.set_params(params)
+        with g.Context(p, g.PREC_F64) as c:
+            c.upload(B0, xb, 0.0, 0.01)
+            info = u.warm_start(c)
+            assert not info.used and info.mesh_ok and info.ratio_applied == info.ratio_cold
+            assert np.array_equal(c.download(g.FIELD_XB), xb)
+
+
+def test_argument_errors(g):
+    with pytest.raises(g.GlemurError):
+        g.UNet(10, 9, 9)
+    p = small_braid(9, 9, 17)
+    B0, xb = g.init_state(p)
+    with g.UNet(9, 9, 13, capacity=1) as u, g.Context(p, g.PREC_F64) as c:
+        c.upload(B0, xb, 0.0, 0.01)
+        with pytest.raises(g.GlemurError):
+            u.add_sample(c)        # grid mismatch
+    with g.UNet(9, 9, 17, capacity=1) as u, g.Context(p, g.PREC_F64) as c:
+        c.upload(B0, xb, 0.0, 0.01)
+        u.add_sample(c)
+        with pytest.raises(g.GlemurError):
+            u.add_sample(c)        # capacity
