@@ -28,6 +28,11 @@ ensemble: 64 seeded braids of the sizes 33x33x97 ... 97x97x289 advanced
 together (one launch per stage for all members); aggregate grid-point
 RK-stages per second.
 
+time_to_relax: wall time from the initial state of the 129 x 129 x 385
+braid to max|JxB| / max(|J||B|) <= 0.03, cold and with the learned warm
+start (U-Net inference, up-sampling, checks and 50 settle attempts
+included).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -139,6 +144,21 @@ def ensemble(g, workloads, n, steps):
             "unit": "grid-pt RK-stages/s", "attempts_total": int(att)}
 
 
+def time_to_relax(g, theta):
+    """Cold start versus the learned warm start on the same braid: wall time
+    to the first attempted step with max|JxB|/max(|J||B|) <= theta."""
+    from glemur_amd import warmstart as ws
+    p = ws.braid(None, ws.FINE)
+    cold = ws.time_to_relax(p, theta, 20000)
+    with g.UNet(*ws.COARSE) as u:
+        ws.load(u)
+        warm = ws.time_to_relax(p, theta, 20000, unet=u, settle=50)
+    return {"theta": theta, "cold_s": cold["seconds"], "cold_attempts": cold["attempts"],
+            "warm_s": warm["seconds"], "warm_attempts": warm["attempts"], "warm_used": warm["warm_used"],
+            "warm_start_ms": warm["warm_ms"], "gain": cold["seconds"] / warm["seconds"] if warm["converged"] else None,
+            "weights": "glemur_amd/data/unet_braid65x65x193.npz (512 seeded 65x65x193 braids, 2000 attempts each)"}
+
+
 def cpu_baseline(p, B0, xb, budget_s=20.0):
     import oracle as o
     threads = min(16, os.cpu_count() or 1)
@@ -172,6 +192,8 @@ def main():
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--relax-seconds", type=float, default=10.0)
     ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0: skip)")
+    ap.add_argument("--ttr", type=float, default=0.03,
+                    help="time-to-relax threshold for the cold/warm-start comparison (0: skip)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick each stage kernel's tile by timing candidates before measuring")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -252,6 +274,8 @@ def main():
         out["relaxation"] = relaxation_probe(g, p, B0, xb, args.relax_seconds)
     if args.ensemble > 0:
         out["ensemble"] = ensemble(g, workloads, args.ensemble, 20)
+    if args.ttr > 0:
+        out["time_to_relax"] = time_to_relax(g, args.ttr)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, B0, xb)
     print(json.dumps(out))

@@ -185,3 +185,18 @@ def test_argument_errors(g):
         u.add_sample(c)
         with pytest.raises(g.GlemurError):
             u.add_sample(c)        # capacity
+
+
+def test_committed_weights_warm_start_the_plain_braid(g):
+    """The trained weights shipped with the package warm-start the plain
+    129 x 129 x 385 braid: the warm state is kept (criterion and energy
+    below the cold state's after 50 settle attempts) and reaches
+    max|JxB|/max(|J||B|) <= 0.03 within 1000 attempted steps (the cold start
+    needs about 3800)."""
+    from glemur_amd import warmstart as ws
+    with g.UNet(*ws.COARSE) as u:
+        ws.load(u)
+        r = ws.time_to_relax(ws.braid(None, ws.FINE), 0.03, 1000, unet=u, settle=50)
+    print("warm time to relax:", r)
+    assert r["warm_used"] and r["converged"] and r["attempts"] <= 1000
+    assert r["ratio_after_settle"] < r["ratio_cold"] and r["energy_warm"] < r["energy_cold"]
