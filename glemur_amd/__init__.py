@@ -26,6 +26,7 @@ TIMING_KINDS = 7      # gm_timing_read: stages 0..5, controller
 GM_ERR_ARG, GM_ERR_HIP, GM_ERR_IO, GM_ERR_UNSUPPORTED, GM_ERR_NAN, GM_ERR_PARSE = -1, -2, -3, -4, -5, -6
 PREC_F64, PREC_F32K = 0, 1
 FIELD_XB, FIELD_B0, FIELD_B, FIELD_DETJAC, FIELD_J, FIELD_GRADP, FIELD_KK, FIELD_XB_NEW, FIELD_XB_LO = range(9)
+FIELD_CELLVOL, FIELD_CONVEXITY, FIELD_WEDGEMIN = 9, 10, 11
 
 
 class Params(C.Structure):
@@ -162,7 +163,7 @@ _SIGS = {
     "gm_get_tile": (C.c_int, [_CTX, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "gm_write_b0": (C.c_int, [C.c_char_p, _P(Params), _D, C.c_int]),
     "gm_dump_state": (C.c_int, [C.c_char_p, _P(Params), C.c_double, C.c_double, C.c_int,
-                                 _D, _D, _D, _D, C.c_int]),
+                                 _D, _D, _D, _D, _D, _D, _D, C.c_int]),
     "gm_read_state": (C.c_int, [C.c_char_p, _P(Params), _D, _D, _D, _D, C.c_int]),
     "gm_write_ts": (C.c_int, [C.c_char_p, _P(Params), _P(Reduction), C.c_double, C.c_double, C.c_int,
                                C.c_double, C.c_int, C.c_int]),
@@ -305,7 +306,8 @@ class Context:
     def download(self, field):
         sizes = {FIELD_XB: 3 * self.p.npad, FIELD_B0: 3 * self.p.npad, FIELD_B: 3 * self.p.npad,
                  FIELD_DETJAC: self.p.npad, FIELD_J: 3 * self.p.ninner, FIELD_GRADP: 3 * self.p.ninner,
-                 FIELD_KK: 18 * self.p.ninner, FIELD_XB_NEW: 3 * self.p.npad, FIELD_XB_LO: 3 * self.p.npad}
+                 FIELD_KK: 18 * self.p.ninner, FIELD_XB_NEW: 3 * self.p.npad, FIELD_XB_LO: 3 * self.p.npad,
+                 FIELD_CELLVOL: self.p.ninner, FIELD_CONVEXITY: self.p.ninner, FIELD_WEDGEMIN: self.p.ninner}
         out = np.empty(sizes[field])
         _check(_lib.gm_download(self._h, field, _dptr(out)))
         return out
@@ -445,11 +447,12 @@ def write_b0(dirpath, p, B0, precision_bits=64):
     _check(_lib.gm_write_b0(str(dirpath).encode(), C.byref(p), _dptr(np.ascontiguousarray(B0)), precision_bits))
 
 
-def dump_state(dirpath, p, t, dt, n, xb, B, J=None, detJac=None, precision_bits=64):
+def dump_state(dirpath, p, t, dt, n, xb, B, J=None, detJac=None, precision_bits=64, cellVol=None,
+               convexity=None, wedgeMin=None):
     nul = C.cast(None, _D)
-    _check(_lib.gm_dump_state(str(dirpath).encode(), C.byref(p), t, dt, n, _dptr(xb), _dptr(B),
-                              _dptr(J) if J is not None else nul,
-                              _dptr(detJac) if detJac is not None else nul, precision_bits))
+    opt = lambda a: _dptr(a) if a is not None else nul
+    _check(_lib.gm_dump_state(str(dirpath).encode(), C.byref(p), t, dt, n, _dptr(xb), _dptr(B), opt(J), opt(detJac),
+                              opt(cellVol), opt(convexity), opt(wedgeMin), precision_bits))
 
 
 def read_state(dirpath, p, precision_bits=64):

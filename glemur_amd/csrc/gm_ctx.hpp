@@ -35,7 +35,8 @@ struct gm_ctx {
     long long ghost_off[6] = {0, 0, 0, 0, 0, 0};
     gmd::StepRecord *log = nullptr;
     int log_cap = 4096;
-    double *red_out = nullptr, *red_host = nullptr;   // 4 doubles each
+    double *red_out = nullptr, *red_host = nullptr;   // 16 doubles each
+    double *extra_part = nullptr;                     // mesh / error diagnostics partials
     double *diag_scratch = nullptr;
     int diag_blocks = 2048;
     int *nanflag = nullptr;

@@ -69,6 +69,18 @@ template <class R>
 void launch_diag(hipStream_t s, const Geo &g, const R *B, const R *J, const double *det, double *scratch,
                  int max_blocks, double *out);
 
+// Mesh and error diagnostics (src/diagnostics.cu:33-110): out[0] min
+// convexity, out[1] min octant wedge, out[2] errB_1ez, out[3] errXb_XbAn,
+// out[4] epsilon*, each only when its flag is set; per-point cell volume,
+// convexity and wedge minimum (inner points, x fastest) when the arrays are
+// given.  part holds extra_diag_parts() doubles.  Returns -1 when nz is too
+// large for epsilon* (more than 3840 planes).
+int extra_diag_parts();
+template <class R>
+int launch_extra_diag(hipStream_t s, const Geo &g, const double *x64, const float *xh, const float *xl, const R *B,
+                      const R *J, double ar, double az, double phi1, int mesh, int errs, int eps, double *cellVol,
+                      double *convex, double *wedge, double *part, double *out);
+
 // ---- layout conversion between host AoS padded and device SoA --------
 // dst[c*P + a] = src[3a + c] (src in double) with an optional hi/lo split.
 void launch_aos_to_soa_f64(hipStream_t s, long long P, const double *src, double *dst);

@@ -102,7 +102,7 @@ extern "C" int gm_write_b0(const char *dir, const gm_params *p, const double *B0
 // dumpState + writeParams, src/io.cu:209-386, 503-538
 extern "C" int gm_dump_state(const char *dir, const gm_params *p, double t, double dt, int n,
                              const double *xb, const double *B, const double *J, const double *detJac,
-                             int precision)
+                             const double *cellVol, const double *convexity, const double *wedgeMin, int precision)
 {
     if (!p || !xb || !B) return fail(GM_ERR_ARG, "gm_dump_state: null pointer");
     bool dbl = precision != 32;
@@ -178,6 +178,22 @@ extern "C" int gm_dump_state(const char *dir, const gm_params *p, double t, doub
         }
         ok &= be.flush(fd);
     }
+    // inner-point scalars, ghosts written as zeros (src/io.cu:334-385)
+    auto inner_scalar = [&](const char *nm, const double *v) {
+        fprintf(fd, "SCALARS %s %s\n", nm, real_str(dbl));
+        fprintf(fd, "LOOKUP_TABLE default\n");
+        for (int k = 0; k < nz + 2; ++k)
+            for (int j = 0; j < ny + 2; ++j)
+                for (int i = 0; i < nx + 2; ++i) {
+                    bool ghost = i == 0 || i == nx + 1 || j == 0 || j == ny + 1 || k == 0 || k == nz + 1;
+                    be.real(ghost ? 0. : v[(i - 1) + (long long)nx * ((j - 1) + (long long)ny * (k - 1))], dbl);
+                    if (be.buf.size() > (1 << 20)) ok &= be.flush(fd);
+                }
+        ok &= be.flush(fd);
+    };
+    if (p->dumpCellVol && cellVol) inner_scalar("cellVol", cellVol);
+    if (p->dumpConvexity && convexity) inner_scalar("convexity", convexity);
+    if (p->dumpWedgeMin && wedgeMin) inner_scalar("wedgeMin", wedgeMin);
     fclose(fd);
     return ok ? GM_OK : fail(GM_ERR_IO, "short write to '" + fn + "'");
 }

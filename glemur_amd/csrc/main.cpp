@@ -77,7 +77,7 @@ int main(int argc, char *argv[])
 
     const long long P3 = 3LL * (p.nx + 2) * (p.ny + 2) * (p.nz + 2);
     const long long N3 = 3LL * p.nx * p.ny * p.nz;
-    std::vector<double> B0(P3), xb(P3), B(P3), J(N3), det(P3 / 3);
+    std::vector<double> B0(P3), xb(P3), B(P3), J(N3), det(P3 / 3), cvol(N3 / 3), conv(N3 / 3), wmin(N3 / 3);
     double t = 0, dt = p.dt0;
     gm_reduction red{};
     red.B2res = gm_init_residual(&p);
@@ -123,10 +123,16 @@ int main(int argc, char *argv[])
                 if (gm_download(ctx, GM_FIELD_XB, xb.data()) || gm_download(ctx, GM_FIELD_B, B.data()) ||
                     gm_download(ctx, GM_FIELD_J, J.data()) || gm_download(ctx, GM_FIELD_DETJAC, det.data()))
                     return die("download");
+                if ((p.dumpCellVol && gm_download(ctx, GM_FIELD_CELLVOL, cvol.data())) ||
+                    (p.dumpConvexity && gm_download(ctx, GM_FIELD_CONVEXITY, conv.data())) ||
+                    (p.dumpWedgeMin && gm_download(ctx, GM_FIELD_WEDGEMIN, wmin.data())))
+                    return die("download");
                 if (dump)
                     gm_dump_state(".", &p, t, dt, (int)ceil((t - dt) / p.dtDump), xb.data(), B.data(), J.data(),
-                                  det.data(), 64);
-                if (save) gm_dump_state(".", &p, t, dt, -1, xb.data(), B.data(), J.data(), det.data(), 64);
+                                  det.data(), cvol.data(), conv.data(), wmin.data(), 64);
+                if (save)
+                    gm_dump_state(".", &p, t, dt, -1, xb.data(), B.data(), J.data(), det.data(), cvol.data(),
+                                  conv.data(), wmin.data(), 64);
             }
         } else {
             timeIndex -= 1;

@@ -359,10 +359,11 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     c->ctlh->n_cycle = 1LL << 62;
     c->ctlh->max_attempt = 1LL << 62;
     if ((r = dalloc(&c->log, (size_t)c->log_cap))) return bail(r);
-    if ((r = dalloc(&c->red_out, 8)) || (r = dalloc(&c->diag_scratch, (size_t)6 * c->diag_blocks)) ||
+    if ((r = dalloc(&c->red_out, 16)) || (r = dalloc(&c->diag_scratch, (size_t)6 * c->diag_blocks)) ||
+        (r = dalloc(&c->extra_part, (size_t)gmd::extra_diag_parts())) ||
         (r = dalloc(&c->nanflag, 1)))
         return bail(r);
-    if (hipHostMalloc((void **)&c->red_host, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&c->red_host, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(GM_ERR_HIP, "hipHostMalloc"));
     {
         hipDeviceProp_t pr;
@@ -386,7 +387,7 @@ extern "C" int gm_destroy(gm_ctx *c)
     dfree(c->B0_32); dfree(c->K32); dfree(c->B32); dfree(c->J32); dfree(c->G32);
     dfree(c->det); dfree(c->epsx); dfree(c->epsy); dfree(c->epsz);
     dfree(c->ctl); dfree(c->part_delta); dfree(c->part_diag); dfree(c->tab_dev); dfree(c->log); dfree(c->ghost);
-    dfree(c->red_out); dfree(c->diag_scratch); dfree(c->nanflag);
+    dfree(c->red_out); dfree(c->diag_scratch); dfree(c->nanflag); dfree(c->extra_part);
     if (c->ctlh) (void)hipHostFree(c->ctlh);
     if (c->red_host) (void)hipHostFree(c->red_host);
     for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -464,6 +465,8 @@ extern "C" int gm_synchronize(gm_ctx *c)
     return GM_OK;
 }
 
+static int extra_diag(gm_ctx *c, int mesh, int errs, int eps, double *cellVol, double *convex, double *wedge);
+
 extern "C" int gm_download(gm_ctx *c, int field, double *host)
 {
     if (!c || !host) return fail(GM_ERR_ARG, "gm_download: null pointer");
@@ -509,6 +512,16 @@ extern "C" int gm_download(gm_ctx *c, int field, double *host)
         if ((r = dalloc(&tmp, n))) return r;
         if (f64) gmd::launch_padded_to_inner_aos<double>(s, c->geo, field == GM_FIELD_J ? c->J64 : c->G64, tmp);
         else gmd::launch_padded_to_inner_aos<float>(s, c->geo, field == GM_FIELD_J ? c->J32 : c->G32, tmp);
+        break;
+    }
+    case GM_FIELD_CELLVOL:
+    case GM_FIELD_CONVEXITY:
+    case GM_FIELD_WEDGEMIN: {
+        n = (size_t)c->N;
+        if ((r = dalloc(&tmp, n))) return r;
+        r = extra_diag(c, 0, 0, 0, field == GM_FIELD_CELLVOL ? tmp : nullptr, field == GM_FIELD_CONVEXITY ? tmp : nullptr,
+                       field == GM_FIELD_WEDGEMIN ? tmp : nullptr);
+        if (r) { dfree(tmp); return r; }
         break;
     }
     case GM_FIELD_KK: {
@@ -623,6 +636,23 @@ extern "C" int gm_accept(gm_ctx *c)
     return push_ctl(c);
 }
 
+// Mesh / error diagnostics of the current state into red_out[8..12] and,
+// when given, per-point device arrays (inner points).
+static int extra_diag(gm_ctx *c, int mesh, int errs, int eps, double *cellVol, double *convex, double *wedge)
+{
+    const bool f64 = c->prec == GM_PREC_F64;
+    const int k = c->cur;
+    const int rc = f64 ? gmd::launch_extra_diag<double>(c->stream, c->geo, c->X64[k], nullptr, nullptr, c->B64, c->J64,
+                                                        c->p.ar, c->p.az, c->p.phi1, mesh, errs, eps, cellVol, convex,
+                                                        wedge, c->extra_part, c->red_out + 8)
+                       : gmd::launch_extra_diag<float>(c->stream, c->geo, nullptr, c->Xh[k], c->Xl[k], c->B32, c->J32,
+                                                       c->p.ar, c->p.az, c->p.phi1, mesh, errs, eps, cellVol, convex,
+                                                       wedge, c->extra_part, c->red_out + 8);
+    if (rc) return fail(GM_ERR_UNSUPPORTED, "epsilonStar: more than 3840 planes along z");
+    HIPCHK(hipGetLastError());
+    return GM_OK;
+}
+
 extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
 {
     if (!c || !red) return fail(GM_ERR_ARG, "gm_reductions: null pointer");
@@ -630,7 +660,12 @@ extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
         gmd::launch_diag<double>(c->stream, c->geo, c->B64, c->J64, c->det, c->diag_scratch, c->diag_blocks, c->red_out);
     else
         gmd::launch_diag<float>(c->stream, c->geo, c->B32, c->J32, c->det, c->diag_scratch, c->diag_blocks, c->red_out);
-    HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
+    const gm_params &p = c->p;
+    const int mesh = p.redConvex || p.redWedgeMin, errs = p.redErrB_1ez || p.redErrXb_XbAn, eps = p.redEpsilonStar;
+    if (mesh || errs || eps) {
+        if (int r = extra_diag(c, mesh, errs, eps, nullptr, nullptr, nullptr)) return r;
+    }
+    HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 16 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     memset(red, 0, sizeof *red);
     red->B2 = c->red_host[0];
@@ -638,6 +673,15 @@ extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
     red->JxB_B2Max = c->red_host[2];
     red->JxB_JBMax = c->red_host[3];
     red->JxBMax_JBMax = c->red_host[5] > 0 ? c->red_host[4] / c->red_host[5] : 0.0;
+    if (mesh) {
+        red->convex = c->red_host[8];
+        red->wedgeMin = c->red_host[9];
+    }
+    if (errs) {
+        red->errB_1ez = c->red_host[10];
+        red->errXb_XbAn = c->red_host[11];
+    }
+    if (eps) red->epsilonStar = c->red_host[12];
     red->B2res = gm_init_residual(&c->p);
     return GM_OK;
 }

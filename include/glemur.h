@@ -123,6 +123,9 @@ typedef struct gm_step_info {
 #define GM_FIELD_KK      6   /* 18N six stage slopes, inner AoS, back to back */
 #define GM_FIELD_XB_NEW  7   /* 3P  candidate positions of the last attempt */
 #define GM_FIELD_XB_LO   8   /* 3P  compensation word of the positions (F32K; zeros in F64) */
+#define GM_FIELD_CELLVOL 9   /*  N  signed cell volume, forward differences (src/maths.cu:443-494) */
+#define GM_FIELD_CONVEXITY 10 /* N  +1 / -1: all 8 octant wedges positive (src/maths.cu:573-640) */
+#define GM_FIELD_WEDGEMIN 11 /*  N  smallest oriented octant wedge (src/maths.cu:497-570) */
 
 typedef struct gm_ctx gm_ctx;
 
@@ -328,7 +331,8 @@ int gm_write_b0(const char *dir, const gm_params *p, const double *B0, int preci
  * Arrays in reference layout; J is inner AoS; NULL skips optional fields. */
 int gm_dump_state(const char *dir, const gm_params *p, double t, double dt, int n,
                   const double *xb, const double *B, const double *J,
-                  const double *detJac, int precision);
+                  const double *detJac, const double *cellVol, const double *convexity,
+                  const double *wedgeMin, int precision);
 /* readState (src/io.cu:390-499): xb, t, dt from save.vtk, B0 from B0.vtk. */
 int gm_read_state(const char *dir, const gm_params *p, double *xb, double *B0,
                   double *t, double *dt, int precision);

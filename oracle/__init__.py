@@ -82,6 +82,10 @@ _SIGS = {
     "or_jxb_b2_max": (C.c_double, [_PP, _D, _D]),
     "or_jxb_jb_max": (C.c_double, [_PP, _D, _D]),
     "or_jxbmax_jbmax": (C.c_double, [_PP, _D, _D]),
+    "or_mesh": (None, [_PP, _D, _D, _D, _D, _D, _D]),
+    "or_err_b_1ez": (C.c_double, [_PP, _D]),
+    "or_err_xb_xban": (C.c_double, [_PP, _D]),
+    "or_epsilon_star": (C.c_double, [_PP, _D, _D, _D]),
     "or_state_create": (C.c_int, [_PP, _D, _D, C.POINTER(State)]),
     "or_state_free": (None, [C.POINTER(State)]),
     "or_attempt": (C.c_double, [_PP, C.POINTER(State), C.POINTER(C.c_int)]),
@@ -193,6 +197,29 @@ def set_threads(n):
 
 def num_threads():
     return _lib.or_num_threads()
+
+
+def mesh(p, xb):
+    """(cellVol, convexity, wedgeMin) per inner point [N] and the grid minima
+    (convexity, wedgeMin)."""
+    n = p.nx * p.ny * p.nz
+    cv, co, wm = np.zeros(n), np.zeros(n), np.zeros(n)
+    cmin, wmin = C.c_double(), C.c_double()
+    _lib.or_mesh(C.byref(p), dp(np.ascontiguousarray(xb)), dp(cv), dp(co), dp(wm), C.byref(cmin), C.byref(wmin))
+    return cv, co, wm, cmin.value, wmin.value
+
+
+def err_b_1ez(p, B):
+    return _lib.or_err_b_1ez(C.byref(p), dp(np.ascontiguousarray(B)))
+
+
+def err_xb_xban(p, xb):
+    return _lib.or_err_xb_xban(C.byref(p), dp(np.ascontiguousarray(xb)))
+
+
+def epsilon_star(p, xb, B, J):
+    return _lib.or_epsilon_star(C.byref(p), dp(np.ascontiguousarray(xb)), dp(np.ascontiguousarray(B)),
+                                dp(np.ascontiguousarray(J)))
 
 
 # ---------------------------------------------------------------- U-Net

@@ -938,3 +938,120 @@ int or_step(const or_params *p, or_state *s, double *dt_used, double *maxDelta, 
     if (md > 0) s->dt = dt * pow(0.9 * p->maxError / md, 0.2);
     return acc;
 }
+
+/* ---- mesh and error diagnostics ------------------------------------ */
+
+/* dirCellVol, gridWedgeMin, gridConvexity (src/maths.cu:443-640) per inner
+ * point (x fastest, any array may be NULL), and the grid minima of
+ * convexity and wedgeMin (src/diagnostics.cu:84-110: -max(-v)). */
+void or_mesh(const or_params *p, const double *xb, double *cellVol, double *convex, double *wedge,
+             double *convex_min, double *wedge_min)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double cmin = 0, wmin = 0;
+    int first = 1;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double *x = xb + 3 * pid(p, i + 1, j + 1, k + 1);
+                double wm = 0, cv = 1;
+                for (int u = 0; u <= 2; u += 2)
+                    for (int v = 0; v <= 2; v += 2)
+                        for (int w = 0; w <= 2; w += 2) {
+                            const double *a = xb + 3 * pid(p, i + u, j + 1, k + 1);
+                            const double *b = xb + 3 * pid(p, i + 1, j + v, k + 1);
+                            const double *c = xb + 3 * pid(p, i + 1, j + 1, k + w);
+                            double d1[3], d2[3], d3[3], cr[3];
+                            for (int l = 0; l < 3; l++) {
+                                d1[l] = a[l] - x[l];
+                                d2[l] = b[l] - x[l];
+                                d3[l] = c[l] - x[l];
+                            }
+                            cross3(d1, d2, cr);
+                            double wc = cr[0] * d3[0] + cr[1] * d3[1] + cr[2] * d3[2];
+                            if ((wc < 0) && (u + v + w == 2 || u + v + w == 6)) cv = -1;
+                            if ((wc > 0) && (u + v + w == 0 || u + v + w == 4)) cv = -1;
+                            if (u + v + w == 0) wm = -wc;
+                            if (u + v + w == 0 || u + v + w == 4) wc = -wc;
+                            if (wc < wm) wm = wc;
+                        }
+                double d1[3], d2[3], d3[3], cr[3];
+                const double *a = xb + 3 * pid(p, i + 2, j + 1, k + 1);
+                const double *b = xb + 3 * pid(p, i + 1, j + 2, k + 1);
+                const double *c = xb + 3 * pid(p, i + 1, j + 1, k + 2);
+                for (int l = 0; l < 3; l++) {
+                    d1[l] = a[l] - x[l];
+                    d2[l] = b[l] - x[l];
+                    d3[l] = c[l] - x[l];
+                }
+                cross3(d1, d2, cr);
+                long q = iid(p, i, j, k);
+                if (cellVol) cellVol[q] = cr[0] * d3[0] + cr[1] * d3[1] + cr[2] * d3[2];
+                if (convex) convex[q] = cv;
+                if (wedge) wedge[q] = wm;
+                if (first || cv < cmin) cmin = cv;
+                if (first || wm < wmin) wmin = wm;
+                first = 0;
+            }
+    if (convex_min) *convex_min = cmin;
+    if (wedge_min) *wedge_min = wmin;
+}
+
+/* B_1ez + sumGlobal (src/maths.cu:294-330, src/diagnostics.cu:51-60) */
+double or_err_b_1ez(const or_params *p, const double *B)
+{
+    double s = 0;
+    for (int k = 0; k < p->nz; k++)
+        for (int j = 0; j < p->ny; j++)
+            for (int i = 0; i < p->nx; i++) {
+                const double *b = B + 3 * pid(p, i + 1, j + 1, k + 1);
+                s += b[0] * b[0] + b[1] * b[1] + (b[2] - 1.) * (b[2] - 1.);
+            }
+    return sqrt(s) / sqrt((double)p->nx * p->ny * p->nz);
+}
+
+/* xb_XbAn + sumGlobal (src/maths.cu:334-377, src/diagnostics.cu:62-71):
+ * distance to the analytic twist x_an of the "analytic" initial field */
+double or_err_xb_xban(const or_params *p, const double *xb)
+{
+    double s = 0;
+    for (int k = 0; k < p->nz; k++)
+        for (int j = 0; j < p->ny; j++)
+            for (int i = 0; i < p->nx; i++) {
+                const double *v = xb + 3 * pid(p, i + 1, j + 1, k + 1);
+                double x = i * p->dx + p->Ox, y = j * p->dy + p->Oy, z = k * p->dz + p->Oz;
+                double e = exp(-(x * x + y * y) / (p->ar * p->ar) - z * z / (p->az * p->az));
+                double d0 = v[0] - cos(e * p->phi1) * x - sin(e * p->phi1) * y;
+                double d1 = v[1] + sin(e * p->phi1) * x - cos(e * p->phi1) * y;
+                double d2 = v[2] - z;
+                s += d0 * d0 + d1 * d1 + d2 * d2;
+            }
+    return sqrt(s) / sqrt((double)p->nx * p->ny * p->nz);
+}
+
+/* JB_B2 + epsilonStar + findDevMax (src/maths.cu:204-290,
+ * src/diagnostics.cu:33-49): along the column i = nx/2, j = ny/2, the
+ * largest sqrt2 |a_n - a_k| / |z_n - z_k| over all pairs (0 where z is
+ * equal), a = J.B/B^2, z the position's z.  For nz <= 256 the reference
+ * covers exactly these pairs; above, its tiling leaves entries unwritten. */
+double or_epsilon_star(const or_params *p, const double *xb, const double *B, const double *J)
+{
+    const int nz = p->nz, i = p->nx / 2, j = p->ny / 2;
+    const double SQRT2 = 1.414213562373095;   /* src/global.h:18 */
+    double *a = (double *)malloc(sizeof(double) * nz), *z = (double *)malloc(sizeof(double) * nz);
+    for (int k = 0; k < nz; k++) {
+        const double *b = B + 3 * pid(p, i + 1, j + 1, k + 1);
+        const double *jj = J + 3 * iid(p, i, j, k);
+        a[k] = (jj[0] * b[0] + jj[1] * b[1] + jj[2] * b[2]) / (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        z[k] = xb[2 + 3 * pid(p, i + 1, j + 1, k + 1)];
+    }
+    double m = 0;
+    for (int k = 0; k < nz; k++)
+        for (int n = 0; n < nz; n++) {
+            double v = z[n] != z[k] ? SQRT2 * fabs((a[n] - a[k]) / (z[n] - z[k])) : 0;
+            if (m < v) m = v;
+        }
+    free(a);
+    free(z);
+    return m;
+}

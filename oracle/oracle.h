@@ -122,6 +122,11 @@ double or_jxb_jb_max(const or_params *p, const double *B, const double *J);
 /* max |JxB| / max (|J||B|) (0 when J vanishes): the relaxation criterion
  * (no reference counterpart). */
 double or_jxbmax_jbmax(const or_params *p, const double *B, const double *J);
+void or_mesh(const or_params *p, const double *xb, double *cellVol, double *convex, double *wedge,
+             double *convex_min, double *wedge_min);
+double or_err_b_1ez(const or_params *p, const double *B);
+double or_err_xb_xban(const or_params *p, const double *xb);
+double or_epsilon_star(const or_params *p, const double *xb, const double *B, const double *J);
 
 /* Whole-step driver state (src/main.cu:259-384). */
 typedef struct or_state {

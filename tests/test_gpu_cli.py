@@ -32,8 +32,9 @@ PARAMS = """&comp
 /
 &io
     nTs = 2, dtDump = 0.05, dtSave = 0.1
-    dumpJ = t, dumpDetJac = t
-    redB2 = t, redJMax = t, redJxB_B2Max = t
+    dumpJ = t, dumpDetJac = t, dumpCellVol = t, dumpConvexity = t, dumpWedgeMin = t
+    redB2 = t, redJMax = t, redJxB_B2Max = t, redConvex = t, redWedgeMin = t, redEpsilonStar = t
+    redErrB_1ez = t
 /
 """
 
@@ -62,6 +63,9 @@ def test_run_directory_contract(require_gpu, tmp_path, prec):
     assert (tmp_path / "data" / "save.vtk").exists()
     dumps = [f for f in os.listdir(tmp_path / "data") if f.startswith("dump")]
     assert dumps
+    raw = (tmp_path / "data" / dumps[0]).read_bytes()
+    for name in (b"SCALARS cellVol", b"SCALARS convexity", b"SCALARS wedgeMin"):
+        assert name in raw
     head, rows = ts_rows(tmp_path)
     assert head[:5] == ["#", "it", "t", "dt", "maxDelta"] or head[:4] == ["#it", "t", "dt", "maxDelta"]
     # nTs = 2: a row on every second accepted step (time index 0, 2, ...)
@@ -69,8 +73,11 @@ def test_run_directory_contract(require_gpu, tmp_path, prec):
     assert its == list(range(0, 30, 2))
     t = [float(r[1]) for r in rows]
     assert all(b > a for a, b in zip(t, t[1:]))
-    b2 = [float(r[head.index("B2") - (1 if head[0] == "#" else 0)]) for r in rows]
+    col = lambda name: [float(r[head.index(name) - (1 if head[0] == "#" else 0)]) for r in rows]
+    b2 = col("B2")
     assert all(b <= a for a, b in zip(b2, b2[1:]))
+    assert all(v == 1.0 for v in col("convex")) and all(v > 0 for v in col("wedgeMin"))
+    assert all(v > 0 for v in col("epsilonStar")) and all(v > 0 for v in col("errB_1ez"))
 
 
 def test_restart_continues(require_gpu, tmp_path):

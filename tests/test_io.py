@@ -76,15 +76,16 @@ def test_b0_vtk_layout(tmp_path, bits):
 
 @pytest.mark.parametrize("bits", [64, 32])
 def test_dump_vtk_layout(tmp_path, bits):
-    p = small_params(dumpJ=True, dumpDetJac=True)
+    p = small_params(dumpJ=True, dumpDetJac=True, dumpCellVol=True, dumpConvexity=True, dumpWedgeMin=True)
     os.makedirs(tmp_path / "data")
     rng = np.random.default_rng(0)
     xb = rng.standard_normal(3 * p.npad)
     B = rng.standard_normal(3 * p.npad)
     J = rng.standard_normal(3 * p.ninner)
     det = rng.random(p.npad)
+    mesh = {k: rng.standard_normal(p.ninner) for k in ("cellVol", "convexity", "wedgeMin")}
     t, dt = 1.25, 3.5e-3
-    g.dump_state(tmp_path, p, t, dt, 4, xb, B, J, det, bits)
+    g.dump_state(tmp_path, p, t, dt, 4, xb, B, J, det, bits, **mesh)
     fdt, rs = (">f8", "double") if bits == 64 else (">f4", "float")
     r = Reader(tmp_path / "data" / "dump4.vtk")
     assert r.line() == "# vtk DataFile Version 2.0"
@@ -106,6 +107,13 @@ def test_dump_vtk_layout(tmp_path, bits):
     assert r.line() == f"SCALARS detJac {rs}"
     assert r.line() == "LOOKUP_TABLE default"
     np.testing.assert_array_equal(r.block(fdt, p.npad), det.astype(fdt))
+    for name in ("cellVol", "convexity", "wedgeMin"):     # src/io.cu:334-385: ghosts are zeros
+        assert r.line() == f"SCALARS {name} {rs}"
+        assert r.line() == "LOOKUP_TABLE default"
+        v = r.block(fdt, p.npad).reshape(p.nz + 2, p.ny + 2, p.nx + 2)
+        want = np.zeros_like(v)
+        want[1:-1, 1:-1, 1:-1] = mesh[name].reshape(p.nz, p.ny, p.nx).astype(fdt)
+        np.testing.assert_array_equal(v, want)
     assert r.done()
 
 
