@@ -233,6 +233,72 @@ __global__ void k_set_grid_peri(Geo g, double *X, int face)
 }
 
 // ---------------------------------------------------------------- JStokes
+// JClassic (src/current.cu:10-219): J = curl_x of B = jac B0 / det by the
+// chain rule, with central differences on the initial grid X:
+// dB_l/dX_a = sum_b d2x_l/dX_a dX_b B0_b / det + jac_lb dB0_b/dX_a / det
+// + jac_lb B0_b d(1/det)/dX_a, then dB/dx = dB/dX jac^-1.  The reference's
+// operation order is kept (bitwise FP64 parity with the oracle).
+__global__ void __launch_bounds__(kBlock) k_jclassic(Geo g, const double *X, const double *B0, const double *det,
+                                                     double *J)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long N = (long long)g.nx * g.ny * g.nz;
+    if (t >= N) return;
+    int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long long)g.nx * g.ny)) + 1;
+    const long long a0 = gidx(g, i, j, k);
+    const long long st[3] = {1, g.sx, g.sxy};
+    const double h1[3] = {g.dx1, g.dy1, g.dz1};
+    double x0[3], b0[3], jac[3][3], d2[3][3][3], dB0[3][3], dD1[3];
+    ld3(X, g.P, a0, x0);
+    ld3(B0, g.P, a0, b0);
+    for (int m = 0; m < 3; ++m) {
+        double xp[3], xm[3], bp[3], bm[3];
+        ld3(X, g.P, a0 + st[m], xp);
+        ld3(X, g.P, a0 - st[m], xm);
+        ld3(B0, g.P, a0 + st[m], bp);
+        ld3(B0, g.P, a0 - st[m], bm);
+        for (int l = 0; l < 3; ++l) {
+            jac[l][m] = (xp[l] - xm[l]) * h1[m] / 2;
+            d2[l][m][m] = (xp[l] - 2 * x0[l] + xm[l]) * h1[m] * h1[m];
+            dB0[l][m] = (bp[l] - bm[l]) * h1[m] / 2;
+        }
+        dD1[m] = (1 / det[a0 + st[m]] - 1 / det[a0 - st[m]]) * h1[m] / 2;
+    }
+    for (int m = 0; m < 3; ++m)
+        for (int n = m + 1; n < 3; ++n) {
+            double pp[3], pm[3], mp[3], mm[3];
+            ld3(X, g.P, a0 + st[m] + st[n], pp);
+            ld3(X, g.P, a0 + st[m] - st[n], pm);
+            ld3(X, g.P, a0 - st[m] + st[n], mp);
+            ld3(X, g.P, a0 - st[m] - st[n], mm);
+            for (int l = 0; l < 3; ++l) {
+                // the y-z term subtracts its middle corners in the other order
+                d2[l][m][n] = (m == 1 ? pp[l] - mp[l] - pm[l] + mm[l] : pp[l] - pm[l] - mp[l] + mm[l]) * h1[m] *
+                              h1[n] / 4;
+                d2[l][n][m] = d2[l][m][n];
+            }
+        }
+    const double det1 = 1 / det[a0];
+    double jac1[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const int c1 = (c + 1) % 3, c2 = (c + 2) % 3, r1 = (r + 1) % 3, r2 = (r + 2) % 3;
+            jac1[r][c] = (jac[c1][r1] * jac[c2][r2] - jac[c1][r2] * jac[c2][r1]) * det1;
+        }
+    double Jv[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double T[3];
+            for (int l = 0; l < 3; ++l)
+                T[l] = d2[l][a][b] * b0[b] * det1 + jac[l][b] * dB0[b][a] * det1 + jac[l][b] * b0[b] * dD1[a];
+            for (int c = 0; c < 3; ++c) {
+                const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                Jv[c] += jac1[a][c1] * T[c2] - jac1[a][c2] * T[c1];
+            }
+        }
+    for (int c = 0; c < 3; ++c) J[c * g.P + a0] = Jv[c];
+}
+
 __global__ void __launch_bounds__(kBlock) k_jstokes(Geo g, const double *X, const double *B, double *J)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -552,6 +618,10 @@ void launch_set_grid_peri(hipStream_t s, const Geo &g, double *X)
     if (g.zPeri) hipLaunchKernelGGL(k_set_grid_peri, dim3(blocks_for((long long)(g.nx + 2) * (g.ny + 2))), dim3(kBlock), 0, s, g, X, 2);
     if (g.yPeri) hipLaunchKernelGGL(k_set_grid_peri, dim3(blocks_for((long long)(g.nx + 2) * (g.nz + 2))), dim3(kBlock), 0, s, g, X, 1);
     if (g.xPeri) hipLaunchKernelGGL(k_set_grid_peri, dim3(blocks_for((long long)(g.ny + 2) * (g.nz + 2))), dim3(kBlock), 0, s, g, X, 0);
+}
+void launch_jclassic(hipStream_t s, const Geo &g, const double *X, const double *B0, const double *det, double *J)
+{
+    hipLaunchKernelGGL(k_jclassic, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, X, B0, det, J);
 }
 void launch_jstokes(hipStream_t s, const Geo &g, const double *X, const double *B, double *J)
 {

@@ -286,8 +286,15 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     if (pp->nx < 2 || pp->ny < 2 || pp->nz < 2) return fail(GM_ERR_ARG, "grid needs at least 2 inner points per axis");
     if (pp->inertia) return fail(GM_ERR_UNSUPPORTED, "inertia = t is not supported (magnetofrictional equation only)");
     if (pp->jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver = t is not supported");
-    if (!gm::starts(pp->jMethod, "Stokes "))
-        return fail(GM_ERR_UNSUPPORTED, std::string("jMethod '") + std::string(pp->jMethod, 29) + "': only Stokes is built");
+    const bool classic = gm::starts(pp->jMethod, "Classic ");
+    if (!gm::starts(pp->jMethod, "Stokes ") && !classic)
+        return fail(GM_ERR_UNSUPPORTED, std::string("jMethod '") + std::string(pp->jMethod, 29) +
+                                            "': Stokes and Classic are built");
+    if (classic && precision != GM_PREC_F64)
+        return fail(GM_ERR_UNSUPPORTED, "jMethod = Classic runs on the FP64 per-kernel path only");
+    if (classic && (pp->xPeri || pp->yPeri || pp->zPeri))
+        return fail(GM_ERR_UNSUPPORTED, "jMethod = Classic with periodic faces: the reference leaves detJac unset "
+                                        "on periodic ghost faces, which Classic reads");
     if (pp->pressure && !gm::starts(pp->pMethod, "Classic "))
         return fail(GM_ERR_UNSUPPORTED, "pMethod: only Classic is built");
     int ndev = 0;
@@ -578,7 +585,10 @@ extern "C" int gm_current(gm_ctx *c)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     if (int r = need_f64(c, "gm_current")) return r;
-    gmd::launch_jstokes(c->stream, c->geo, c->X64[c->cur], c->B64, c->J64);
+    if (gm::starts(c->p.jMethod, "Classic "))
+        gmd::launch_jclassic(c->stream, c->geo, c->X64[c->cur], c->B0_64, c->det, c->J64);
+    else
+        gmd::launch_jstokes(c->stream, c->geo, c->X64[c->cur], c->B64, c->J64);
     return GM_OK;
 }
 extern "C" int gm_gradp(gm_ctx *c)
@@ -691,12 +701,34 @@ static int check_fused(gm_ctx *c)
 {
     if (any_peri(c->p))
         return fail(GM_ERR_UNSUPPORTED, "periodic boundaries run through the per-kernel path (gm_b_jacb0 ...)");
+    if (gm::starts(c->p.jMethod, "Classic "))
+        return fail(GM_ERR_UNSUPPORTED, "jMethod = Classic runs through the per-kernel path (gm_current ...)");
+    return GM_OK;
+}
+
+// FP64 contexts the fused stage kernels do not cover (periodic faces,
+// jMethod = Classic) step through the per-kernel sequence
+static bool per_kernel_path(gm_ctx *c)
+{
+    return c->prec == GM_PREC_F64 && (any_peri(c->p) || gm::starts(c->p.jMethod, "Classic "));
+}
+
+// Fields of the current state through the per-kernel sequence
+// (B_JacB0, setBbound, setPeri(B), current, gradP): the path of FP64
+// contexts with periodic faces or jMethod = Classic.
+static int fields_per_kernel(gm_ctx *c)
+{
+    int r;
+    if ((r = gm_b_jacb0(c)) || (r = gm_set_bbound(c)) || (r = gm_set_peri(c, GM_FIELD_B)) || (r = gm_current(c)))
+        return r;
+    if (c->p.pressure && (r = gm_gradp(c))) return r;
     return GM_OK;
 }
 
 extern "C" int gm_fields(gm_ctx *c)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
+    if (per_kernel_path(c)) return fields_per_kernel(c);
     if (int r = check_fused(c)) return r;
     gmd::FusedArgs A = fused_args(c, -1, 1, 0, 0);
     gmd::launch_fused(c->stream, A, c->prec == GM_PREC_F32K);
@@ -719,10 +751,91 @@ static void fill_info(gm_ctx *c, gm_step_info *info)
     info->n_accept = h.n_accept;
 }
 
+// One attempted step of an FP64 context on the per-kernel path (periodic
+// faces, jMethod = Classic): the
+// reference's per-kernel sequence (src/main.cu:259-383) on the GPU, with
+// the device controller's rules applied on the host (host pow, as the
+// reference).  The stage positions update x in place (src/main.cu:288), so
+// a rejected attempt continues from that x, as the reference does; the
+// criterion of the starting state is evaluated before the stages.
+// Expects c->ctlh synchronised; pushes it back.
+static int step_per_kernel(gm_ctx *c)
+{
+    gmd::Ctl &h = *c->ctlh;
+    if (h.done) return GM_OK;
+    int r;
+    if ((r = fields_per_kernel(c))) return r;
+    gmd::launch_diag<double>(c->stream, c->geo, c->B64, c->J64, c->det, c->diag_scratch, c->diag_blocks, c->red_out);
+    HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 6 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double *d = c->red_host;
+    const double crit = d[5] > 0 ? d[4] / d[5] : 0.0;
+    gmd::StepRecord rec{};
+    rec.t = h.t;
+    rec.dt_used = h.dt;
+    h.energy = d[0];
+    h.jmax = d[1];
+    h.jxb_b2 = d[2];
+    h.ratio = crit;
+    h.ratio_pt = d[3];
+    h.diag_valid = 1;
+    rec.ratio = crit;
+    rec.ratio_pt = d[3];
+    rec.energy = d[0];
+    rec.jxb_b2 = d[2];
+    rec.jmax = d[1];
+    auto log = [&](void) -> int {
+        HIPCHK(hipMemcpy(c->log + (h.n_attempt % c->log_cap), &rec, sizeof rec, hipMemcpyHostToDevice));
+        return GM_OK;
+    };
+    if (h.stop_ratio > 0 && crit <= h.stop_ratio) {
+        h.converged = 1;
+        h.done = 1;
+        h.stop_reason = 3;
+        h.accepted = 0;
+        rec.accepted = -1;
+        if ((r = log())) return r;
+        return push_ctl(c);
+    }
+    const double dt = h.dt;
+    for (int n = 0; n < 6; ++n) {
+        if (n > 0 && (r = fields_per_kernel(c))) return r;
+        if ((r = gm_rk_kk(c, n, dt))) return r;
+    }
+    double md = 0;
+    if ((r = gm_rk_xnewstar(c, &md))) return r;
+    int nan = 0;
+    HIPCHK(hipMemcpy(&nan, c->nanflag, sizeof nan, hipMemcpyDeviceToHost));
+    h.nan_seen |= nan;
+    rec.nan_seen = nan;
+    rec.maxDelta = md;
+    h.dt_used = dt;
+    h.maxDelta = md;
+    const int acc = md < c->p.maxError;
+    h.accepted = acc;
+    if (acc) {
+        h.t += dt;
+        h.cur = (h.cur + 1) % 3;
+        h.n_accept += 1;
+    }
+    if (md > 0) h.dt = dt * std::pow(0.9 * c->p.maxError / md, 0.2);
+    rec.accepted = acc;
+    rec.t = h.t;
+    if ((r = log())) return r;
+    h.n_attempt += 1;
+    if (h.n_accept >= h.n_cycle) { h.done = 1; h.stop_reason = 1; }
+    else if (!(h.dt > c->p.dtMin)) { h.done = 1; h.stop_reason = 2; }
+    else if (h.n_attempt >= h.max_attempt) { h.done = 1; h.stop_reason = 4; }
+    c->cur = h.cur;
+    return push_ctl(c);
+}
+
+
 extern "C" int gm_step(gm_ctx *c, gm_step_info *info)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
-    if (int r = check_fused(c)) return r;
+    if (!per_kernel_path(c))
+        if (int r = check_fused(c)) return r;
     int r = sync_ctl(c);
     if (r) return r;
     gmd::Ctl &h = *c->ctlh;
@@ -734,7 +847,11 @@ extern "C" int gm_step(gm_ctx *c, gm_step_info *info)
     h.n_cycle = 1LL << 62;
     h.max_attempt = 1LL << 62;
     if ((r = push_ctl(c))) return r;
-    enqueue_step(c, 1, 1);
+    if (per_kernel_path(c)) {
+        if ((r = step_per_kernel(c))) return r;
+    } else {
+        enqueue_step(c, 1, 1);
+    }
     HIPCHK(hipGetLastError());
     if ((r = sync_ctl(c))) return r;
     if (info) fill_info(c, info);
@@ -745,7 +862,8 @@ extern "C" int gm_step(gm_ctx *c, gm_step_info *info)
 extern "C" int gm_run(gm_ctx *c, const gm_run_opts *o, gm_run_stats *st)
 {
     if (!c || !o) return fail(GM_ERR_ARG, "gm_run: null pointer");
-    if (int r = check_fused(c)) return r;
+    if (!per_kernel_path(c))
+        if (int r = check_fused(c)) return r;
     int r = sync_ctl(c);
     if (r) return r;
     gmd::Ctl &h = *c->ctlh;
@@ -760,7 +878,7 @@ extern "C" int gm_run(gm_ctx *c, const gm_run_opts *o, gm_run_stats *st)
     h.max_attempt = h.n_attempt + (o->max_attempts > 0 ? o->max_attempts : (1LL << 40));
     if ((r = push_ctl(c))) return r;
     int batch = std::max(1, o->batch);
-    bool graph = o->use_graph && !c->timing;
+    bool graph = o->use_graph && !c->timing && !per_kernel_path(c);
     auto t0 = std::chrono::steady_clock::now();
     if (graph && (!c->gexec || c->gbatch != batch)) {
         if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -773,7 +891,11 @@ extern "C" int gm_run(gm_ctx *c, const gm_run_opts *o, gm_run_stats *st)
         (void)hipGraphDestroy(gr);
         c->gbatch = batch;
     }
-    while (true) {
+    while (per_kernel_path(c)) {
+        if ((r = step_per_kernel(c))) return r;
+        if (h.done) break;
+    }
+    while (!per_kernel_path(c)) {
         if (graph)
             HIPCHK(hipGraphLaunch(c->gexec, c->stream));
         else

@@ -892,6 +892,8 @@ void or_state_free(or_state *s)
     memset(s, 0, sizeof *s);
 }
 
+void or_jclassic(const or_params *p, const double *xb, const double *B0, const double *detJac, double *J);
+
 static int any_peri(const or_params *p)
 {
     return p->xPeri || p->yPeri || p->zPeri;
@@ -902,7 +904,10 @@ void or_fields(const or_params *p, or_state *s)
     or_b_jacb0(p, s->xb, s->B0, s->B, s->detJac);
     or_update_bbound(p, s->xb, s->B0, s->B, s->detJac);
     if (any_peri(p)) or_set_peri(p, s->B);
-    or_jstokes(p, s->xb, s->B, s->J);
+    if (strncmp(p->jMethod, "Classic ", 8) == 0)
+        or_jclassic(p, s->xb, s->B0, s->detJac, s->J);
+    else
+        or_jstokes(p, s->xb, s->B, s->J);
 }
 
 /* src/main.cu:260-319 */
@@ -1054,4 +1059,67 @@ double or_epsilon_star(const or_params *p, const double *xb, const double *B, co
     free(a);
     free(z);
     return m;
+}
+
+/* JClassic, src/current.cu:10-219: curl of B = jac B0/det by the chain
+ * rule with central differences (reference operation order). */
+void or_jclassic(const or_params *p, const double *xb, const double *B0, const double *detJac, double *J)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const double h1[3] = {p->dx1, p->dy1, p->dz1};
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int c0[3] = {i + 1, j + 1, k + 1};
+                double jac[3][3], d2[3][3][3], dB0[3][3], dD1[3], jac1[3][3];
+#define XP(di, dj, dk) (xb + 3 * pid(p, c0[0] + (di), c0[1] + (dj), c0[2] + (dk)))
+#define BP(di, dj, dk) (B0 + 3 * pid(p, c0[0] + (di), c0[1] + (dj), c0[2] + (dk)))
+#define DP(di, dj, dk) (detJac[pid(p, c0[0] + (di), c0[1] + (dj), c0[2] + (dk))])
+                const int e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+                const double *x0 = XP(0, 0, 0), *b0 = BP(0, 0, 0);
+                for (int m = 0; m < 3; m++) {
+                    const double *xp = XP(e[m][0], e[m][1], e[m][2]), *xm = XP(-e[m][0], -e[m][1], -e[m][2]);
+                    const double *bp = BP(e[m][0], e[m][1], e[m][2]), *bm = BP(-e[m][0], -e[m][1], -e[m][2]);
+                    for (int l = 0; l < 3; l++) {
+                        jac[l][m] = (xp[l] - xm[l]) * h1[m] / 2;
+                        d2[l][m][m] = (xp[l] - 2 * x0[l] + xm[l]) * h1[m] * h1[m];
+                        dB0[l][m] = (bp[l] - bm[l]) * h1[m] / 2;
+                    }
+                    dD1[m] = (1 / DP(e[m][0], e[m][1], e[m][2]) - 1 / DP(-e[m][0], -e[m][1], -e[m][2])) * h1[m] / 2;
+                }
+                /* mixed derivatives in the reference's term order */
+                for (int l = 0; l < 3; l++) {
+                    d2[l][0][1] = (XP(1, 1, 0)[l] - XP(1, -1, 0)[l] - XP(-1, 1, 0)[l] + XP(-1, -1, 0)[l]) * h1[0] * h1[1] / 4;
+                    d2[l][0][2] = (XP(1, 0, 1)[l] - XP(1, 0, -1)[l] - XP(-1, 0, 1)[l] + XP(-1, 0, -1)[l]) * h1[0] * h1[2] / 4;
+                    d2[l][1][2] = (XP(0, 1, 1)[l] - XP(0, -1, 1)[l] - XP(0, 1, -1)[l] + XP(0, -1, -1)[l]) * h1[1] * h1[2] / 4;
+                    d2[l][1][0] = d2[l][0][1];
+                    d2[l][2][0] = d2[l][0][2];
+                    d2[l][2][1] = d2[l][1][2];
+                }
+#undef XP
+#undef BP
+#undef DP
+                const double det1 = 1 / detJac[pid(p, c0[0], c0[1], c0[2])];
+                jac1[0][0] = (jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1]) * det1;
+                jac1[0][1] = (jac[0][2] * jac[2][1] - jac[0][1] * jac[2][2]) * det1;
+                jac1[0][2] = (jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1]) * det1;
+                jac1[1][0] = (jac[1][2] * jac[2][0] - jac[1][0] * jac[2][2]) * det1;
+                jac1[1][1] = (jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0]) * det1;
+                jac1[1][2] = (jac[0][2] * jac[1][0] - jac[0][0] * jac[1][2]) * det1;
+                jac1[2][0] = (jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0]) * det1;
+                jac1[2][1] = (jac[0][1] * jac[2][0] - jac[0][0] * jac[2][1]) * det1;
+                jac1[2][2] = (jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0]) * det1;
+                double *Jq = J + 3 * iid(p, i, j, k);
+                Jq[0] = Jq[1] = Jq[2] = 0;
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) {
+                        double T[3];
+                        for (int l = 0; l < 3; l++)
+                            T[l] = d2[l][a][b] * b0[b] * det1 + jac[l][b] * dB0[b][a] * det1 + jac[l][b] * b0[b] * dD1[a];
+                        Jq[0] += jac1[a][1] * T[2] - jac1[a][2] * T[1];
+                        Jq[1] += jac1[a][2] * T[0] - jac1[a][0] * T[2];
+                        Jq[2] += jac1[a][0] * T[1] - jac1[a][1] * T[0];
+                    }
+            }
 }

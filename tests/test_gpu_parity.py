@@ -217,3 +217,56 @@ def test_periodic_per_kernel_bitwise(g, peri):
         md_g = ctx.rk_xnewstar()
         assert md_g == md_o
         assert_bitwise(ctx.download(g.FIELD_XB_NEW), sim.arr("xb_new"), "xb_new")
+
+
+@pytest.mark.parametrize("peri", [("xPeri",), ("yPeri",), ("zPeri",), ("xPeri", "yPeri", "zPeri")])
+def test_periodic_step_and_run_bitwise(g, peri):
+    """gm_step / gm_run on contexts with periodic faces (the per-kernel
+    sequence, host-side control): 30 attempts bit for bit with the oracle,
+    dt included (both use the host pow); gm_fields + gm_reductions on the
+    result equal the oracle's; gm_run honours its attempt limit."""
+    p = small_braid(**{k: True for k in peri})
+    B0, xb = evolved_state(p, 0)
+    sim = oracle_sim(p, B0, xb)
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, p.dt0)
+        sim.set_dt(p.dt0)
+        n_rej = 0
+        for it in range(30):
+            s = ctx.step()
+            acc, dtu, md, _ = sim.step()
+            assert s.dt_used == dtu and s.maxDelta == md and bool(s.accepted) == acc, it
+            assert s.dt == sim.dt, it
+            n_rej += not acc
+        assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), "positions")
+        ctx.fields()
+        r = ctx.reductions()
+        sim.fields()
+        assert r.JMax == sim.jmax() and r.B2 == pytest.approx(sim.b2(), rel=1e-13)
+        st = ctx.run(max_attempts=7, batch=4)
+        assert st.n_attempt == 37 and st.stopped and st.ratio > 0
+        log = ctx.step_log(7)
+        assert len(log) == 7 and all(x.dt_used > 0 and x.ratio > 0 for x in log)
+
+
+def test_jclassic_fields_and_steps_bitwise(g):
+    """jMethod = Classic (src/current.cu:10-219) on the FP64 per-kernel
+    path: J on a distorted grid and 20 attempted steps (gm_step) bit for
+    bit with the oracle; F32K contexts refuse it."""
+    p = small_braid()
+    p.set_str("jMethod", "Classic")
+    B0, xb = evolved_state(p, 2)
+    sim = oracle_sim(p, B0, xb)
+    sim.fields()
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, 0.01)
+        ctx.fields()
+        assert_bitwise(ctx.download(g.FIELD_J), sim.arr("J"), "J (Classic)")
+        sim.set_dt(0.01)
+        for it in range(20):
+            s = ctx.step()
+            acc, dtu, md, _ = sim.step()
+            assert s.maxDelta == md and bool(s.accepted) == acc and s.dt == sim.dt, it
+        assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), "positions (Classic)")
+    with pytest.raises(g.GlemurError):
+        g.Context(p, g.PREC_F32K)
