@@ -156,7 +156,7 @@ def time_to_relax(g, theta):
     return {"theta": theta, "cold_s": cold["seconds"], "cold_attempts": cold["attempts"],
             "warm_s": warm["seconds"], "warm_attempts": warm["attempts"], "warm_used": warm["warm_used"],
             "warm_start_ms": warm["warm_ms"], "gain": cold["seconds"] / warm["seconds"] if warm["converged"] else None,
-            "weights": "glemur_amd/data/unet_braid65x65x193.npz (512 seeded 65x65x193 braids, 2000 attempts each)"}
+            "weights": "glemur_amd/data/unet_braid65x65x193.npz (2048 seeded 65x65x193 braids, 2000 attempts each)"}
 
 
 def cpu_baseline(p, B0, xb, budget_s=20.0):
