@@ -74,6 +74,29 @@ struct Lds {
     }
 };
 
+// Raw buffer view of one float array: loads and stores take a 32-bit byte
+// offset per lane and a 32-bit byte offset held in a scalar register, so
+// the SoA component stride costs no per-lane 64-bit address arithmetic
+// (4% off every stage at 129 x 129 x 385); offsets past the array read 0
+// and drop writes.
+struct BufF {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ float ld(int voff, int soff) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    }
+    __device__ __forceinline__ void st(float v, int voff, int soff) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    }
+};
+// 0x00020000: the resource word-3 format of the gfx9 family
+__device__ __forceinline__ BufF buf_of(const float *p, long long nfloats)
+{
+    const long long bytes = nfloats * 4;
+    return BufF{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0,
+                                                  (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL), 0x00020000)};
+}
 template <int NT>
 constexpr size_t lds_bytes32()
 {
@@ -94,7 +117,7 @@ __device__ __forceinline__ double det32(const float a[3][3])
 __device__ __forceinline__ double cauchy32(const float jac[3][3], const float b0[3], int normal, float B[3])
 {
     double det = det32(jac);
-    float rd = 1.0f / (float)det;
+    const float rd = __builtin_amdgcn_rcpf((float)det);   // 1 ulp: below the FP32 rounding of B
 #pragma unroll
     for (int l = 0; l < 3; ++l)
         B[l] = l == normal ? b0[l] : (jac[l][0] * b0[0] + jac[l][1] * b0[1] + jac[l][2] * b0[2]) * rd;
@@ -163,9 +186,13 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const long long abase = (long long)gi + g.sx * (long long)gj;
     const int ii = gi - 1, jj = gj - 1;
     const float eps_xy = (g.epsilonProf && inJ) ? (float)(A.epsx[ii] * A.epsy[jj]) : 1.0f;
-    const float *Xh0 = A.Xh[cur], *Xl0 = A.Xl[cur];
-    const float *B0 = A.B0_32;
-    const float *K = A.K32;
+    // buffer views: positions, B0, one per slope slot; the component l of a
+    // point at padded index a sits at byte 4a (lane) + 4lP (scalar)
+    const BufF bXh = buf_of(A.Xh[cur], P3), bXl = buf_of(A.Xl[cur], P3), bB0 = buf_of(A.B0_32, P3);
+    BufF bK[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) bK[m] = buf_of(A.K32 + (long long)(m < 5 ? m : 4) * P3, P3);
+    const int sP = (int)(4 * P);     // byte stride of a component
 
     // ---- registers carried between planes
     float b0q[2][3];      // B0 at planes p, p+1
@@ -184,19 +211,19 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     auto ghost_pt = [&](int q) { return gx || gy || q == 0 || q == g.nz + 1; };
 
     auto load_plane = [&](int q, In &in) {
-        const long long a = abase + g.sxy * (long long)q;
+        const int va = 4 * (int)(abase + g.sxy * (long long)q);
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
-            in.x[l] = Xh0[l * P + a];
-            in.lo[l] = Xl0[l * P + a];
-            in.b0[l] = B0[l * P + a];
+            in.x[l] = bXh.ld(va, l * sP);
+            in.lo[l] = bXl.ld(va, l * sP);
+            in.b0[l] = bB0.ld(va, l * sP);
         }
         if (!ghost_pt(q)) {
 #pragma unroll
             for (int m = 0; m < 5; ++m)
                 if (m < n)
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) in.k[m][l] = K[m * P3 + l * P + a];
+                    for (int l = 0; l < 3; ++l) in.k[m][l] = bK[m].ld(va, l * sP);
         }
     };
 
@@ -395,7 +422,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             }
             {
                 // det[dx dy dz] = 8 hx hy hz det(jac), det(jac) from FP64
-                const float rD = (float)(1.0 / (dq[0] * (8.0 * g.dx * g.dy * g.dz)));
+                const float rD = __builtin_amdgcn_rcpf((float)(dq[0] * (8.0 * g.dx * g.dy * g.dz)));
 #pragma unroll
                 for (int l = 0; l < 3; ++l) Jv[l] *= rD;
             }
@@ -443,7 +470,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 L.ldx(pj + 1, t - RW, pc); L.ldx(pj - 1, t - RW, pd);
 #pragma unroll
                 for (int l = 0; l < 3; ++l) d2[l][1][2] = d2[l][2][1] = (dif(pa, pb, l) - dif(pc, pd, l)) * h1[1] * h1[2] * 0.25f;
-                gradp_core(jac, d2, (float)(1 / detv), (float)g.beta, gv);
+                gradp_core(jac, d2, __builtin_amdgcn_rcpf((float)detv), (float)g.beta, gv);
             }
 #pragma unroll
             for (int l = 0; l < 3; ++l) {
@@ -479,13 +506,15 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             }
             if constexpr (!FINAL) {
                 if constexpr (stage >= 0) {
+                    const int va = 4 * (int)a;
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) A.K32[stage * P3 + l * P + a] = kv[l];
+                    for (int l = 0; l < 3; ++l) bK[stage].st(kv[l], va, l * sP);
                 }
             } else {
                 // x_new = x_5 + sum_m (c_m - b5_m) k_m (k_5 included), into
                 // (hi, lo) by a two-sum; error |sum_m (c_m - c*_m) k_m| in FP64
-                float *Xhd = A.Xh[cur ^ 1], *Xld = A.Xl[cur ^ 1];
+                const BufF bXhd = buf_of(A.Xh[cur ^ 1], P3), bXld = buf_of(A.Xl[cur ^ 1], P3);
+                const int va = 4 * (int)a;
                 Pt3 xc;
                 L.ldx(pj, t, xc);
 #pragma unroll
@@ -503,10 +532,10 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                         dmax = nb_max(dmax, dd);
                     } else {
                         // z held: x_5's small part is the old lo word (no slope terms on z)
-                        lo = A.Xl[cur][l * P + a];
+                        lo = bXl.ld(va, l * sP);
                     }
-                    Xhd[l * P + a] = hi;
-                    Xld[l * P + a] = lo;
+                    bXhd.st(hi, va, l * sP);
+                    bXld.st(lo, va, l * sP);
                 }
             }
         }

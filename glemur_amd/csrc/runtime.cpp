@@ -297,6 +297,9 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
                                         "on periodic ghost faces, which Classic reads");
     if (pp->pressure && !gm::starts(pp->pMethod, "Classic "))
         return fail(GM_ERR_UNSUPPORTED, "pMethod: only Classic is built");
+    if (precision == GM_PREC_F32K && 12LL * gm::npad(*pp) >= 0x7fffffffLL)
+        return fail(GM_ERR_UNSUPPORTED, "F32K: the grid exceeds the 32-bit byte offsets of the stage kernels' buffer "
+                                        "accesses (3 (nx+2)(ny+2)(nz+2) floats per array, at most 2 GiB)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(GM_ERR_HIP, "no HIP device");
     gm_ctx *c = new gm_ctx();
