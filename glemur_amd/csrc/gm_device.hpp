@@ -54,6 +54,7 @@ void launch_set_peri(hipStream_t s, const Geo &g, double *F);
 void launch_set_grid_peri(hipStream_t s, const Geo &g, double *X);
 void launch_jstokes(hipStream_t s, const Geo &g, const double *X, const double *B, double *J);
 void launch_jclassic(hipStream_t s, const Geo &g, const double *X, const double *B0, const double *det, double *J);
+void launch_jstokes4th(hipStream_t s, const Geo &g, const double *X, const double *B, double *J);
 void launch_gradp(hipStream_t s, const Geo &g, const double *X, const double *det, double *G);
 void launch_kk(hipStream_t s, const Geo &g, const Tableau *tab, int n, double dt, const double *X,
                const double *B, const double *J, const double *G, double *K, double *Xtmp);

@@ -233,6 +233,74 @@ __global__ void k_set_grid_peri(Geo g, double *X, int face)
 }
 
 // ---------------------------------------------------------------- JStokes
+// JStokes4th (src/current.cu:413-735): per coordinate plane through the
+// point (xy, yz, zx: rows 0, 1, 2 of NA), the octilateral through the 4
+// face and 4 edge neighbours, vertices (a, b) = (-1,-1), (0,-1), (1,-1),
+// (1,0), (1,1), (0,1), (-1,1), (-1,0) along the plane's axes: trapezoid
+// circulation I = sum (B_k + B_k-1)/2 . (x_k - x_k-1), area vector = half
+// the corner cross products plus a quarter of the inner diamond's; then
+// J = NA^-1 I with every cofactor divided by det NA.
+__global__ void __launch_bounds__(kBlock) k_jstokes4th(Geo g, const double *X, const double *B, double *J)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long N = (long long)g.nx * g.ny * g.nz;
+    if (t >= N) return;
+    int i = (int)(t % g.nx) + 1, j = (int)((t / g.nx) % g.ny) + 1, k = (int)(t / ((long long)g.nx * g.ny)) + 1;
+    const long long a0 = gidx(g, i, j, k);
+    const long long st[3] = {1, g.sx, g.sxy};
+    const int ax[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+    const int V[8][2] = {{0, 0}, {1, 0}, {2, 0}, {2, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+    double NA[3][3], II[3];
+    for (int q = 0; q < 3; ++q) {
+        double x[8][3], b[8][3];
+        for (int v = 0; v < 8; ++v) {
+            const long long o = a0 + (V[v][0] - 1) * st[ax[q][0]] + (V[v][1] - 1) * st[ax[q][1]];
+            ld3(X, g.P, o, x[v]);
+            ld3(B, g.P, o, b[v]);
+        }
+        double dx[8][3], dA[3], dB[3], dC[3], dD[3];   // dx[k-1] = x_k - x_(k-1), k = 1..8
+        double I = 0;
+        for (int l = 0; l < 3; ++l) {
+            for (int e = 1; e <= 8; ++e) dx[e - 1][l] = x[e % 8][l] - x[e - 1][l];
+            double sI = (b[1][l] + b[0][l]) / 2 * dx[0][l];
+            for (int e = 2; e <= 8; ++e) sI = sI + (b[e % 8][l] + b[e - 1][l]) / 2 * dx[e - 1][l];
+            I += sI;
+            dA[l] = x[1][l] - x[7][l];
+            dB[l] = x[3][l] - x[1][l];
+            dC[l] = x[5][l] - x[3][l];
+            dD[l] = x[7][l] - x[5][l];
+        }
+        II[q] = I;
+        double c1[3], c2[3], c3[3], c4[3];
+        cross3(dx[1], dx[2], c1);
+        cross3(dx[3], dx[4], c2);
+        cross3(dx[5], dx[6], c3);
+        cross3(dx[7], dx[0], c4);
+        for (int l = 0; l < 3; ++l) NA[q][l] = (c1[l] + c2[l] + c3[l] + c4[l]) / 2;
+        cross3(dA, dB, c1);
+        cross3(dB, dC, c2);
+        cross3(dC, dD, c3);
+        cross3(dD, dA, c4);
+        for (int l = 0; l < 3; ++l) NA[q][l] += (c1[l] + c2[l] + c3[l] + c4[l]) / 4;
+    }
+    const double dN = det3(NA);
+    double A[3][3];
+    A[0][0] = (NA[1][1] * NA[2][2] - NA[1][2] * NA[2][1]) / dN;
+    A[0][1] = (NA[0][2] * NA[2][1] - NA[0][1] * NA[2][2]) / dN;
+    A[0][2] = (NA[0][1] * NA[1][2] - NA[0][2] * NA[1][1]) / dN;
+    A[1][0] = (NA[1][2] * NA[2][0] - NA[1][0] * NA[2][2]) / dN;
+    A[1][1] = (NA[0][0] * NA[2][2] - NA[0][2] * NA[2][0]) / dN;
+    A[1][2] = (NA[0][2] * NA[1][0] - NA[0][0] * NA[1][2]) / dN;
+    A[2][0] = (NA[1][0] * NA[2][1] - NA[1][1] * NA[2][0]) / dN;
+    A[2][1] = (NA[0][1] * NA[2][0] - NA[0][0] * NA[2][1]) / dN;
+    A[2][2] = (NA[0][0] * NA[1][1] - NA[0][1] * NA[1][0]) / dN;
+    for (int l = 0; l < 3; ++l) {
+        double sJ = 0;
+        for (int m = 0; m < 3; ++m) sJ += A[l][m] * II[m];
+        J[l * g.P + a0] = sJ;
+    }
+}
+
 // JClassic (src/current.cu:10-219): J = curl_x of B = jac B0 / det by the
 // chain rule, with central differences on the initial grid X:
 // dB_l/dX_a = sum_b d2x_l/dX_a dX_b B0_b / det + jac_lb dB0_b/dX_a / det
@@ -618,6 +686,10 @@ void launch_set_grid_peri(hipStream_t s, const Geo &g, double *X)
     if (g.zPeri) hipLaunchKernelGGL(k_set_grid_peri, dim3(blocks_for((long long)(g.nx + 2) * (g.ny + 2))), dim3(kBlock), 0, s, g, X, 2);
     if (g.yPeri) hipLaunchKernelGGL(k_set_grid_peri, dim3(blocks_for((long long)(g.nx + 2) * (g.nz + 2))), dim3(kBlock), 0, s, g, X, 1);
     if (g.xPeri) hipLaunchKernelGGL(k_set_grid_peri, dim3(blocks_for((long long)(g.ny + 2) * (g.nz + 2))), dim3(kBlock), 0, s, g, X, 0);
+}
+void launch_jstokes4th(hipStream_t s, const Geo &g, const double *X, const double *B, double *J)
+{
+    hipLaunchKernelGGL(k_jstokes4th, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, X, B, J);
 }
 void launch_jclassic(hipStream_t s, const Geo &g, const double *X, const double *B0, const double *det, double *J)
 {

@@ -286,15 +286,18 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     if (pp->nx < 2 || pp->ny < 2 || pp->nz < 2) return fail(GM_ERR_ARG, "grid needs at least 2 inner points per axis");
     if (pp->inertia) return fail(GM_ERR_UNSUPPORTED, "inertia = t is not supported (magnetofrictional equation only)");
     if (pp->jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver = t is not supported");
-    const bool classic = gm::starts(pp->jMethod, "Classic ");
-    if (!gm::starts(pp->jMethod, "Stokes ") && !classic)
+    const bool classic = gm::starts(pp->jMethod, "Classic "), stokes4 = gm::starts(pp->jMethod, "Stokes4th ");
+    if (!gm::starts(pp->jMethod, "Stokes ") && !classic && !stokes4)
         return fail(GM_ERR_UNSUPPORTED, std::string("jMethod '") + std::string(pp->jMethod, 29) +
-                                            "': Stokes and Classic are built");
-    if (classic && precision != GM_PREC_F64)
-        return fail(GM_ERR_UNSUPPORTED, "jMethod = Classic runs on the FP64 per-kernel path only");
+                                            "': Stokes, Stokes4th and Classic are built");
+    if ((classic || stokes4) && precision != GM_PREC_F64)
+        return fail(GM_ERR_UNSUPPORTED, "jMethod = Classic / Stokes4th run on the FP64 per-kernel path only");
     if (classic && (pp->xPeri || pp->yPeri || pp->zPeri))
         return fail(GM_ERR_UNSUPPORTED, "jMethod = Classic with periodic faces: the reference leaves detJac unset "
                                         "on periodic ghost faces, which Classic reads");
+    if (stokes4 && (pp->xPeri || pp->yPeri || pp->zPeri))
+        return fail(GM_ERR_UNSUPPORTED, "jMethod = Stokes4th with periodic faces: the octilaterals read B on "
+                                        "ghost edges, which the periodic copies do not define");
     if (pp->pressure && !gm::starts(pp->pMethod, "Classic "))
         return fail(GM_ERR_UNSUPPORTED, "pMethod: only Classic is built");
     if (precision == GM_PREC_F32K && 12LL * gm::npad(*pp) >= 0x7fffffffLL)
@@ -590,6 +593,8 @@ extern "C" int gm_current(gm_ctx *c)
     if (int r = need_f64(c, "gm_current")) return r;
     if (gm::starts(c->p.jMethod, "Classic "))
         gmd::launch_jclassic(c->stream, c->geo, c->X64[c->cur], c->B0_64, c->det, c->J64);
+    else if (gm::starts(c->p.jMethod, "Stokes4th "))
+        gmd::launch_jstokes4th(c->stream, c->geo, c->X64[c->cur], c->B64, c->J64);
     else
         gmd::launch_jstokes(c->stream, c->geo, c->X64[c->cur], c->B64, c->J64);
     return GM_OK;
@@ -704,16 +709,16 @@ static int check_fused(gm_ctx *c)
 {
     if (any_peri(c->p))
         return fail(GM_ERR_UNSUPPORTED, "periodic boundaries run through the per-kernel path (gm_b_jacb0 ...)");
-    if (gm::starts(c->p.jMethod, "Classic "))
-        return fail(GM_ERR_UNSUPPORTED, "jMethod = Classic runs through the per-kernel path (gm_current ...)");
+    if (!gm::starts(c->p.jMethod, "Stokes "))
+        return fail(GM_ERR_UNSUPPORTED, "jMethod Classic / Stokes4th run through the per-kernel path (gm_current ...)");
     return GM_OK;
 }
 
 // FP64 contexts the fused stage kernels do not cover (periodic faces,
-// jMethod = Classic) step through the per-kernel sequence
+// jMethod = Classic or Stokes4th) step through the per-kernel sequence
 static bool per_kernel_path(gm_ctx *c)
 {
-    return c->prec == GM_PREC_F64 && (any_peri(c->p) || gm::starts(c->p.jMethod, "Classic "));
+    return c->prec == GM_PREC_F64 && (any_peri(c->p) || !gm::starts(c->p.jMethod, "Stokes "));
 }
 
 // Fields of the current state through the per-kernel sequence

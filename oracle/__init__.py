@@ -75,6 +75,7 @@ _SIGS = {
     "or_set_grid_peri": (None, [_PP, _D]),
     "or_jstokes": (None, [_PP, _D, _D, _D]),
     "or_jclassic": (None, [_PP, _D, _D, _D, _D]),
+    "or_jstokes4th": (None, [_PP, _D, _D, _D]),
     "or_gradp": (None, [_PP, _D, _D, _D]),
     "or_kk": (None, [_PP, _D, C.c_int, C.c_double, _D, _D, _D, _D, _D, _D]),
     "or_xnewstar": (C.c_double, [_PP, _D, _D, _D, _D, _D, C.POINTER(C.c_int)]),

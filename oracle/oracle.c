@@ -893,6 +893,7 @@ void or_state_free(or_state *s)
 }
 
 void or_jclassic(const or_params *p, const double *xb, const double *B0, const double *detJac, double *J);
+void or_jstokes4th(const or_params *p, const double *xb, const double *B, double *J);
 
 static int any_peri(const or_params *p)
 {
@@ -906,6 +907,8 @@ void or_fields(const or_params *p, or_state *s)
     if (any_peri(p)) or_set_peri(p, s->B);
     if (strncmp(p->jMethod, "Classic ", 8) == 0)
         or_jclassic(p, s->xb, s->B0, s->detJac, s->J);
+    else if (strncmp(p->jMethod, "Stokes4th ", 10) == 0)
+        or_jstokes4th(p, s->xb, s->B, s->J);
     else
         or_jstokes(p, s->xb, s->B, s->J);
 }
@@ -1121,5 +1124,77 @@ void or_jclassic(const or_params *p, const double *xb, const double *B0, const d
                         Jq[1] += jac1[a][2] * T[0] - jac1[a][0] * T[2];
                         Jq[2] += jac1[a][0] * T[1] - jac1[a][1] * T[0];
                     }
+            }
+}
+
+/* JStokes4th, src/current.cu:413-735: the octilateral through the 4 face and
+ * 4 edge neighbours in each coordinate plane (rows xy, yz, zx of NA), the
+ * trapezoid circulation, the area vector (half the corner cross products
+ * plus a quarter of the inner diamond's), J = NA^-1 I with each cofactor
+ * divided by det NA (reference operation order). */
+void or_jstokes4th(const or_params *p, const double *xb, const double *B, double *J)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    static const int ax[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+    static const int V[8][2] = {{0, 0}, {1, 0}, {2, 0}, {2, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                double NA[3][3], II[3];
+                for (int q = 0; q < 3; q++) {
+                    const double *x[8], *b[8];
+                    for (int v = 0; v < 8; v++) {
+                        int o[3] = {0, 0, 0};
+                        o[ax[q][0]] = V[v][0] - 1;
+                        o[ax[q][1]] = V[v][1] - 1;
+                        long g = pid(p, i + o[0], j + o[1], k + o[2]);
+                        x[v] = xb + 3 * g;
+                        b[v] = B + 3 * g;
+                    }
+                    double dx[8][3], dA[3], dB[3], dC[3], dD[3], I = 0;
+                    for (int l = 0; l < 3; l++) {
+                        for (int e = 1; e <= 8; e++) dx[e - 1][l] = x[e % 8][l] - x[e - 1][l];
+                        I += (b[1][l] + b[0][l]) / 2 * dx[0][l] + (b[2][l] + b[1][l]) / 2 * dx[1][l] +
+                             (b[3][l] + b[2][l]) / 2 * dx[2][l] + (b[4][l] + b[3][l]) / 2 * dx[3][l] +
+                             (b[5][l] + b[4][l]) / 2 * dx[4][l] + (b[6][l] + b[5][l]) / 2 * dx[5][l] +
+                             (b[7][l] + b[6][l]) / 2 * dx[6][l] + (b[0][l] + b[7][l]) / 2 * dx[7][l];
+                        dA[l] = x[1][l] - x[7][l];
+                        dB[l] = x[3][l] - x[1][l];
+                        dC[l] = x[5][l] - x[3][l];
+                        dD[l] = x[7][l] - x[5][l];
+                    }
+                    II[q] = I;
+                    double c1[3], c2[3], c3[3], c4[3];
+                    cross3(dx[1], dx[2], c1);
+                    cross3(dx[3], dx[4], c2);
+                    cross3(dx[5], dx[6], c3);
+                    cross3(dx[7], dx[0], c4);
+                    for (int l = 0; l < 3; l++) NA[q][l] = (c1[l] + c2[l] + c3[l] + c4[l]) / 2;
+                    cross3(dA, dB, c1);
+                    cross3(dB, dC, c2);
+                    cross3(dC, dD, c3);
+                    cross3(dD, dA, c4);
+                    for (int l = 0; l < 3; l++) NA[q][l] += (c1[l] + c2[l] + c3[l] + c4[l]) / 4;
+                }
+                double dN = NA[0][0] * NA[1][1] * NA[2][2] + NA[0][1] * NA[1][2] * NA[2][0] +
+                            NA[0][2] * NA[1][0] * NA[2][1] - NA[0][0] * NA[1][2] * NA[2][1] -
+                            NA[0][1] * NA[1][0] * NA[2][2] - NA[0][2] * NA[1][1] * NA[2][0];
+                double A1[3][3];
+                A1[0][0] = (NA[1][1] * NA[2][2] - NA[1][2] * NA[2][1]) / dN;
+                A1[0][1] = (NA[0][2] * NA[2][1] - NA[0][1] * NA[2][2]) / dN;
+                A1[0][2] = (NA[0][1] * NA[1][2] - NA[0][2] * NA[1][1]) / dN;
+                A1[1][0] = (NA[1][2] * NA[2][0] - NA[1][0] * NA[2][2]) / dN;
+                A1[1][1] = (NA[0][0] * NA[2][2] - NA[0][2] * NA[2][0]) / dN;
+                A1[1][2] = (NA[0][2] * NA[1][0] - NA[0][0] * NA[1][2]) / dN;
+                A1[2][0] = (NA[1][0] * NA[2][1] - NA[1][1] * NA[2][0]) / dN;
+                A1[2][1] = (NA[0][1] * NA[2][0] - NA[0][0] * NA[2][1]) / dN;
+                A1[2][2] = (NA[0][0] * NA[1][1] - NA[0][1] * NA[1][0]) / dN;
+                double *Jo = J + 3 * iid(p, i - 1, j - 1, k - 1);
+                for (int l = 0; l < 3; l++) {
+                    double s = 0;
+                    for (int m = 0; m < 3; m++) s += A1[l][m] * II[m];
+                    Jo[l] = s;
+                }
             }
 }

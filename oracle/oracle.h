@@ -102,6 +102,7 @@ void or_set_grid_peri(const or_params *p, double *xb);
 /* src/current.cu:222-409: J (3N, inner AoS). */
 void or_jstokes(const or_params *p, const double *xb, const double *B, double *J);
 void or_jclassic(const or_params *p, const double *xb, const double *B0, const double *detJac, double *J);
+void or_jstokes4th(const or_params *p, const double *xb, const double *B, double *J);
 /* src/gradP.cu:10-156: gradP (3N, inner AoS). */
 void or_gradp(const or_params *p, const double *xb, const double *detJac, double *gradP);
 /* src/rungeKutta.cu:112-247 (magnetofrictional branch): writes kk[n] and

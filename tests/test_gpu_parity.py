@@ -249,24 +249,26 @@ def test_periodic_step_and_run_bitwise(g, peri):
         assert len(log) == 7 and all(x.dt_used > 0 and x.ratio > 0 for x in log)
 
 
-def test_jclassic_fields_and_steps_bitwise(g):
-    """jMethod = Classic (src/current.cu:10-219) on the FP64 per-kernel
-    path: J on a distorted grid and 20 attempted steps (gm_step) bit for
-    bit with the oracle; F32K contexts refuse it."""
+@pytest.mark.parametrize("method", ["Classic", "Stokes4th"])
+def test_jmethod_variants_fields_and_steps_bitwise(g, method):
+    """jMethod = Classic (src/current.cu:10-219) and Stokes4th
+    (src/current.cu:413-735) on the FP64 per-kernel path: J on a distorted
+    grid and 20 attempted steps (gm_step) bit for bit with the oracle; F32K
+    contexts refuse them."""
     p = small_braid()
-    p.set_str("jMethod", "Classic")
+    p.set_str("jMethod", method)
     B0, xb = evolved_state(p, 2)
     sim = oracle_sim(p, B0, xb)
     sim.fields()
     with g.Context(p, g.PREC_F64) as ctx:
         ctx.upload(B0, xb, 0.0, 0.01)
         ctx.fields()
-        assert_bitwise(ctx.download(g.FIELD_J), sim.arr("J"), "J (Classic)")
+        assert_bitwise(ctx.download(g.FIELD_J), sim.arr("J"), f"J ({method})")
         sim.set_dt(0.01)
         for it in range(20):
             s = ctx.step()
             acc, dtu, md, _ = sim.step()
             assert s.maxDelta == md and bool(s.accepted) == acc and s.dt == sim.dt, it
-        assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), "positions (Classic)")
+        assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), f"positions ({method})")
     with pytest.raises(g.GlemurError):
         g.Context(p, g.PREC_F32K)
