@@ -18,6 +18,8 @@ struct Geo {
     int xPeri, yPeri, zPeri;
     int epsilonProf, zUpdate, pressure;
     double beta, maxError, dtMin;
+    int jxbAver;              // J x B averaged with the face neighbours (per-kernel path)
+    double jxbAverWeight;
     DistParams dist;
 };
 

@@ -448,6 +448,28 @@ __global__ void __launch_bounds__(kBlock) k_kk(Geo g, const Tableau *tab, int n,
     ld3(J, g.P, a, jv);
     ld3(B, g.P, a, bv);
     cross3(jv, bv, kv);
+    if (g.jxbAver) {
+        // jxbAver (src/rungeKutta.cu:172-177, 357-443): (1 - w) J x B plus
+        // w/6 J x B of each face neighbour that is an inner point, in the
+        // order -x, +x, -y, +y, -z, +z; the +z neighbour's B is read at its
+        // own padded index (the reference's (nx+1)(ny+1) pitch there is a
+        // known defect, not reproduced)
+        const double w6 = g.jxbAverWeight / 6;
+        for (int l = 0; l < 3; ++l) kv[l] *= (1 - g.jxbAverWeight);
+        const long long st[3] = {1, g.sx, g.sxy};
+        const int id[3] = {i, j, k}, nn[3] = {g.nx, g.ny, g.nz};
+        for (int m = 0; m < 6; ++m) {
+            const int d = m / 2, up = m % 2;
+            double jm[3] = {0, 0, 0}, bm[3] = {0, 0, 0}, c[3];
+            if (up ? id[d] < nn[d] - 1 : id[d] > 0) {
+                const long long o = a + (up ? st[d] : -st[d]);
+                ld3(J, g.P, o, jm);
+                ld3(B, g.P, o, bm);
+            }
+            cross3(jm, bm, c);
+            for (int l = 0; l < 3; ++l) kv[l] += c[l] * w6;
+        }
+    }
     double eps = g.epsilonProf ? eps_axis(i, g.nx) * eps_axis(j, g.ny) * eps_axis(k, g.nz) : 1.0;
     for (int l = 0; l < 3; ++l) {
         kv[l] *= dt * eps;

@@ -57,6 +57,7 @@ gmd::Geo make_geo(const gm_params &p)
     g.xPeri = p.xPeri; g.yPeri = p.yPeri; g.zPeri = p.zPeri;
     g.epsilonProf = p.epsilonProf; g.zUpdate = p.zUpdate; g.pressure = p.pressure;
     g.beta = p.beta; g.maxError = p.maxError; g.dtMin = p.dtMin;
+    g.jxbAver = p.jxbAver; g.jxbAverWeight = p.jxbAverWeight;
     g.dist.code = p.initDistCode;
     g.dist.shearA = p.initShearA; g.dist.shearB = p.initShearB; g.dist.shearK = p.initShearK;
     g.dist.dx = p.dx; g.dist.dy = p.dy; g.dist.Ox = p.Ox; g.dist.Oy = p.Oy;
@@ -285,7 +286,8 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     if (precision != GM_PREC_F64 && precision != GM_PREC_F32K) return fail(GM_ERR_ARG, "unknown precision");
     if (pp->nx < 2 || pp->ny < 2 || pp->nz < 2) return fail(GM_ERR_ARG, "grid needs at least 2 inner points per axis");
     if (pp->inertia) return fail(GM_ERR_UNSUPPORTED, "inertia = t is not supported (magnetofrictional equation only)");
-    if (pp->jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver = t is not supported");
+    if (pp->jxbAver && precision != GM_PREC_F64)
+        return fail(GM_ERR_UNSUPPORTED, "jxbAver = t runs on the FP64 per-kernel path only");
     const bool classic = gm::starts(pp->jMethod, "Classic "), stokes4 = gm::starts(pp->jMethod, "Stokes4th ");
     if (!gm::starts(pp->jMethod, "Stokes ") && !classic && !stokes4)
         return fail(GM_ERR_UNSUPPORTED, std::string("jMethod '") + std::string(pp->jMethod, 29) +
@@ -711,14 +713,16 @@ static int check_fused(gm_ctx *c)
         return fail(GM_ERR_UNSUPPORTED, "periodic boundaries run through the per-kernel path (gm_b_jacb0 ...)");
     if (!gm::starts(c->p.jMethod, "Stokes "))
         return fail(GM_ERR_UNSUPPORTED, "jMethod Classic / Stokes4th run through the per-kernel path (gm_current ...)");
+    if (c->p.jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver runs through the per-kernel path (gm_rk_kk ...)");
     return GM_OK;
 }
 
 // FP64 contexts the fused stage kernels do not cover (periodic faces,
-// jMethod = Classic or Stokes4th) step through the per-kernel sequence
+// jMethod = Classic or Stokes4th, jxbAver) step through the per-kernel
+// sequence
 static bool per_kernel_path(gm_ctx *c)
 {
-    return c->prec == GM_PREC_F64 && (any_peri(c->p) || !gm::starts(c->p.jMethod, "Stokes "));
+    return c->prec == GM_PREC_F64 && (any_peri(c->p) || !gm::starts(c->p.jMethod, "Stokes ") || c->p.jxbAver);
 }
 
 // Fields of the current state through the per-kernel sequence

@@ -713,6 +713,25 @@ void or_kk(const or_params *p, const double b[42], int n, double dt, const doubl
                 long c = pid(p, i + 1, j + 1, k + 1), q = iid(p, i, j, k);
                 double kv[3];
                 cross3(J + 3 * q, B + 3 * c, kv);
+                if (p->jxbAver) {
+                    /* src/rungeKutta.cu:172-177, 357-443, with the +z
+                     * neighbour's B at its own padded index */
+                    const double w6 = p->jxbAverWeight / 6;
+                    for (int l = 0; l < 3; l++) kv[l] *= (1 - p->jxbAverWeight);
+                    const int id[3] = {i, j, k}, nn[3] = {nx, ny, nz};
+                    for (int m = 0; m < 6; m++) {
+                        int d = m / 2, up = m % 2, o[3] = {0, 0, 0};
+                        double jm[3] = {0, 0, 0}, bm[3] = {0, 0, 0}, cc[3];
+                        o[d] = up ? 1 : -1;
+                        if (up ? id[d] < nn[d] - 1 : id[d] > 0) {
+                            const double *js = J + 3 * iid(p, i + o[0], j + o[1], k + o[2]);
+                            const double *bs = B + 3 * pid(p, i + 1 + o[0], j + 1 + o[1], k + 1 + o[2]);
+                            for (int l = 0; l < 3; l++) { jm[l] = js[l]; bm[l] = bs[l]; }
+                        }
+                        cross3(jm, bm, cc);
+                        for (int l = 0; l < 3; l++) kv[l] += cc[l] * w6;
+                    }
+                }
                 double eps = eps_prof(p, i, j, k);
                 for (int l = 0; l < 3; l++) {
                     kv[l] *= dt * eps;

@@ -249,14 +249,18 @@ def test_periodic_step_and_run_bitwise(g, peri):
         assert len(log) == 7 and all(x.dt_used > 0 and x.ratio > 0 for x in log)
 
 
-@pytest.mark.parametrize("method", ["Classic", "Stokes4th"])
+@pytest.mark.parametrize("method", ["Classic", "Stokes4th", "jxbAver"])
 def test_jmethod_variants_fields_and_steps_bitwise(g, method):
-    """jMethod = Classic (src/current.cu:10-219) and Stokes4th
-    (src/current.cu:413-735) on the FP64 per-kernel path: J on a distorted
-    grid and 20 attempted steps (gm_step) bit for bit with the oracle; F32K
-    contexts refuse them."""
+    """jMethod = Classic (src/current.cu:10-219), jMethod = Stokes4th
+    (src/current.cu:413-735) and jxbAver with weight 0.3
+    (src/rungeKutta.cu:357-443) on the FP64 per-kernel path: J on a
+    distorted grid and 20 attempted steps (gm_step) bit for bit with the
+    oracle; F32K contexts refuse them."""
     p = small_braid()
-    p.set_str("jMethod", method)
+    if method == "jxbAver":
+        p.jxbAver, p.jxbAverWeight = True, 0.3
+    else:
+        p.set_str("jMethod", method)
     B0, xb = evolved_state(p, 2)
     sim = oracle_sim(p, B0, xb)
     sim.fields()
