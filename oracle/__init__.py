@@ -60,7 +60,8 @@ class State(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in
                 ("B0", "xb", "B", "detJac", "J", "gradP", "kk", "xb_tmp", "xb_new")] + [
         ("t", C.c_double), ("dt", C.c_double),
-        ("a", C.c_double * 6), ("b", C.c_double * 42), ("c", C.c_double * 6), ("cs", C.c_double * 6)]
+        ("a", C.c_double * 6), ("b", C.c_double * 42), ("c", C.c_double * 6), ("cs", C.c_double * 6)] + [
+        (n, C.POINTER(C.c_double)) for n in ("uu", "uu_tmp", "uu_new")]
 
 
 _D = C.POINTER(C.c_double)
@@ -79,6 +80,10 @@ _SIGS = {
     "or_gradp": (None, [_PP, _D, _D, _D]),
     "or_kk": (None, [_PP, _D, C.c_int, C.c_double, _D, _D, _D, _D, _D, _D]),
     "or_xnewstar": (C.c_double, [_PP, _D, _D, _D, _D, _D, C.POINTER(C.c_int)]),
+    "or_kk_inertia": (None, [_PP, _D, C.c_int, C.c_double, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
+    "or_xnewstar_inertia": (C.c_double, [_PP, _D, _D, _D, _D, _D, _D, _D, C.POINTER(C.c_int)]),
+    "or_u2": (C.c_double, [_PP, _D, _D]),
+    "or_umax": (C.c_double, [_PP, _D]),
     "or_b2": (C.c_double, [_PP, _D, _D]),
     "or_jmax": (C.c_double, [_PP, _D]),
     "or_jxb_b2_max": (C.c_double, [_PP, _D, _D]),
@@ -147,8 +152,12 @@ class Sim:
 
     def arr(self, name):
         n = {"B0": 3 * self.p.npad, "xb": 3 * self.p.npad, "B": 3 * self.p.npad, "detJac": self.p.npad,
-             "J": 3 * self.p.ninner, "gradP": 3 * self.p.ninner, "kk": 18 * self.p.ninner,
-             "xb_tmp": 3 * self.p.npad, "xb_new": 3 * self.p.npad}[name]
+             "J": 3 * self.p.ninner, "gradP": 3 * self.p.ninner,
+             "kk": (36 if self.p.inertia else 18) * self.p.ninner,
+             "xb_tmp": 3 * self.p.npad, "xb_new": 3 * self.p.npad,
+             "uu": 3 * self.p.ninner, "uu_tmp": 3 * self.p.ninner, "uu_new": 3 * self.p.ninner}[name]
+        if name.startswith("uu") and not self.p.inertia:
+            raise ValueError("the velocity exists with inertia only")
         return np.ctypeslib.as_array(getattr(self.s, name), shape=(n,))
 
     @property
@@ -182,6 +191,12 @@ class Sim:
 
     def jmax(self):
         return _lib.or_jmax(C.byref(self.p), self.s.J)
+
+    def u2(self):
+        return _lib.or_u2(C.byref(self.p), self.s.uu, self.s.detJac)
+
+    def umax(self):
+        return _lib.or_umax(C.byref(self.p), self.s.uu)
 
     def jxb_b2_max(self):
         return _lib.or_jxb_b2_max(C.byref(self.p), self.s.B, self.s.J)

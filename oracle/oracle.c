@@ -779,6 +779,107 @@ double or_xnewstar(const or_params *p, const double c[6], const double cs[6], co
     return mx;
 }
 
+/* src/rungeKutta.cu:112-247, inertia branch (:195-212, 216-246): the
+ * position slope k_x = dt u and the velocity slope
+ * k_u = dt det (J x B) - dt nu u det - dt gradP, six values per point in
+ * kk[n] ([stage][point][x0 x1 x2 u0 u1 u2]); position and velocity are
+ * advanced to the next stage in xb_tmp and uu_tmp.  No friction profile
+ * and no J x B averaging in this branch. */
+void or_kk_inertia(const or_params *p, const double b[42], int n, double dt, const double *xb, const double *B,
+                   const double *J, const double *gradP, const double *detJac, const double *uu, double *kk,
+                   double *xb_tmp, double *uu_tmp)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const long N = ninner(p);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                long c = pid(p, i + 1, j + 1, k + 1), q = iid(p, i, j, k);
+                const double *u = uu + 3 * q;
+                double ku[3], *ks = kk + 6 * q + 6 * N * n;
+                cross3(J + 3 * q, B + 3 * c, ku);
+                for (int l = 0; l < 3; l++) {
+                    ks[l] = dt * u[l];
+                    ku[l] *= dt * detJac[c];
+                    ku[l] -= dt * p->nu * u[l] * detJac[c];
+                    if (p->pressure) ku[l] += -dt * gradP[l + 3 * q];
+                    ks[3 + l] = ku[l];
+                }
+                for (int l = 0; l < 3; l++) {
+                    double x = xb[l + 3 * c], v = u[l];
+                    for (int m = 0; m < n + 1; m++) {
+                        const double w = b[m + (n + 1) * 6] - b[m + n * 6];
+                        x += w * kk[l + 6 * q + 6 * N * m];
+                        v += w * kk[3 + l + 6 * q + 6 * N * m];
+                    }
+                    xb_tmp[l + 3 * c] = x;
+                    uu_tmp[l + 3 * q] = v;
+                }
+            }
+}
+
+/* src/rungeKutta.cu:251-353 with inertia: x5 / x4 from the position slopes
+ * (the error is measured on the positions only), u5 from the velocity
+ * slopes into uu_new. */
+double or_xnewstar_inertia(const or_params *p, const double c[6], const double cs[6], const double *xb,
+                           const double *uu, const double *kk, double *xb_new, double *uu_new, int *nan_seen)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const long N = ninner(p);
+    double mx = 0;
+    int nan = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                long g = pid(p, i + 1, j + 1, k + 1), q = iid(p, i, j, k);
+                for (int l = 0; l < 3; l++) {
+                    double xn = xb[l + 3 * g], xs = xb[l + 3 * g];
+                    if (l < 2 || p->zUpdate)
+                        for (int m = 0; m < 6; m++) {
+                            xn += c[m] * kk[l + 6 * q + 6 * N * m];
+                            xs += cs[m] * kk[l + 6 * q + 6 * N * m];
+                        }
+                    double un = uu[l + 3 * q];
+                    for (int m = 0; m < 6; m++) un += c[m] * kk[3 + l + 6 * q + 6 * N * m];
+                    xb_new[l + 3 * g] = xn;
+                    uu_new[l + 3 * q] = un;
+                    double d = fabs(xn - xs);
+                    if (d != d) nan = 1;
+                    if (d > mx) mx = d;
+                }
+            }
+    if (nan_seen) *nan_seen = nan;
+    return mx;
+}
+
+/* U2_det + sumGlobal, src/maths.cu:683-717, src/diagnostics.cu:114-122:
+ * sum of |u|^2 det with the boundary weights 1/2^(#faces), times
+ * dx dy dz.  Summed serially in (k,j,i) order. */
+double or_u2(const or_params *p, const double *uu, const double *detJac)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    double s = 0;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double *u = uu + 3 * iid(p, i, j, k);
+                double v = (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * detJac[pid(p, i + 1, j + 1, k + 1)];
+                int bound = (i == 0) + (j == 0) + (k == 0) + (i == nx - 1) + (j == ny - 1) + (k == nz - 1);
+                bound = 1 << bound;
+                v /= bound;
+                s += v;
+            }
+    return s * p->dx * p->dy * p->dz;
+}
+
+/* mag + findDevMax, src/maths.cu:157-164, src/diagnostics.cu:124-130:
+ * NaN-blind max of |u| over inner points. */
+double or_umax(const or_params *p, const double *uu)
+{
+    return or_jmax(p, uu);
+}
+
 /* B2_det + sumGlobal, src/maths.cu:647-680, src/diagnostics.cu:73-82.
  * Summed serially in (k,j,i) order. */
 double or_b2(const or_params *p, const double *B, const double *detJac)
@@ -886,10 +987,18 @@ int or_state_create(const or_params *p, const double *B0, const double *xb, or_s
     s->detJac = (double *)calloc(npad(p), sizeof(double));
     s->J = (double *)calloc(N3, sizeof(double));
     s->gradP = (double *)calloc(N3, sizeof(double));
-    s->kk = (double *)calloc(6 * N3, sizeof(double));
+    s->kk = (double *)calloc((p->inertia ? 12 : 6) * N3, sizeof(double));
     s->xb_tmp = (double *)malloc(P3 * sizeof(double));
     s->xb_new = (double *)malloc(P3 * sizeof(double));
-    if (!s->B0 || !s->xb || !s->B || !s->detJac || !s->J || !s->gradP || !s->kk || !s->xb_tmp || !s->xb_new) {
+    if (p->inertia) {
+        /* the velocity starts at rest (uInit = 'nil'; the reference leaves
+         * h.uu as allocated, src/init.cu:358-378, src/memory.cu:34) */
+        s->uu = (double *)calloc(N3, sizeof(double));
+        s->uu_tmp = (double *)calloc(N3, sizeof(double));
+        s->uu_new = (double *)calloc(N3, sizeof(double));
+    }
+    if (!s->B0 || !s->xb || !s->B || !s->detJac || !s->J || !s->gradP || !s->kk || !s->xb_tmp || !s->xb_new ||
+        (p->inertia && (!s->uu || !s->uu_tmp || !s->uu_new))) {
         or_state_free(s);
         return -1;
     }
@@ -908,6 +1017,7 @@ void or_state_free(or_state *s)
 {
     free(s->B0); free(s->xb); free(s->B); free(s->detJac); free(s->J);
     free(s->gradP); free(s->kk); free(s->xb_tmp); free(s->xb_new);
+    free(s->uu); free(s->uu_tmp); free(s->uu_new);
     memset(s, 0, sizeof *s);
 }
 
@@ -939,11 +1049,17 @@ double or_attempt(const or_params *p, or_state *s, int *nan_seen)
     for (int n = 0; n < 6; n++) {
         or_fields(p, s);
         if (p->pressure) or_gradp(p, s->xb, s->detJac, s->gradP);
-        or_kk(p, s->b, n, s->dt, s->xb, s->B, s->J, s->gradP, s->kk, s->xb_tmp);
+        if (p->inertia)
+            or_kk_inertia(p, s->b, n, s->dt, s->xb, s->B, s->J, s->gradP, s->detJac, s->uu, s->kk, s->xb_tmp,
+                          s->uu_tmp);
+        else
+            or_kk(p, s->b, n, s->dt, s->xb, s->B, s->J, s->gradP, s->kk, s->xb_tmp);
         if (any_peri(p)) or_set_grid_peri(p, s->xb_tmp);
         memcpy(s->xb, s->xb_tmp, P3 * sizeof(double));
+        if (p->inertia) memcpy(s->uu, s->uu_tmp, 3 * ninner(p) * sizeof(double));
     }
-    double md = or_xnewstar(p, s->c, s->cs, s->xb, s->kk, s->xb_new, nan_seen);
+    double md = p->inertia ? or_xnewstar_inertia(p, s->c, s->cs, s->xb, s->uu, s->kk, s->xb_new, s->uu_new, nan_seen)
+                           : or_xnewstar(p, s->c, s->cs, s->xb, s->kk, s->xb_new, nan_seen);
     if (any_peri(p)) or_set_grid_peri(p, s->xb_new);
     return md;
 }
@@ -960,6 +1076,7 @@ int or_step(const or_params *p, or_state *s, double *dt_used, double *maxDelta, 
     if (acc) {
         s->t += dt;
         memcpy(s->xb, s->xb_new, P3 * sizeof(double));
+        if (p->inertia) memcpy(s->uu, s->uu_new, 3 * ninner(p) * sizeof(double));
         or_fields(p, s);
     }
     if (md > 0) s->dt = dt * pow(0.9 * p->maxError / md, 0.2);

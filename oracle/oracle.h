@@ -115,6 +115,20 @@ void or_kk(const or_params *p, const double b[42], int n, double dt,
 double or_xnewstar(const or_params *p, const double c[6], const double cs[6],
                    const double *xb, const double *kk, double *xb_new, int *nan_seen);
 
+/* src/rungeKutta.cu:112-247 (inertia branch): kk[n] holds six values per
+ * point (position slope dt u, velocity slope); writes the inner points of
+ * xb_tmp and uu_tmp (3N, inner AoS). */
+void or_kk_inertia(const or_params *p, const double b[42], int n, double dt, const double *xb, const double *B,
+                   const double *J, const double *gradP, const double *detJac, const double *uu, double *kk,
+                   double *xb_tmp, double *uu_tmp);
+/* src/rungeKutta.cu:251-353 (inertia): xb_new and uu_new; the error max is
+ * taken over the positions only. */
+double or_xnewstar_inertia(const or_params *p, const double c[6], const double cs[6], const double *xb,
+                           const double *uu, const double *kk, double *xb_new, double *uu_new, int *nan_seen);
+/* U2 (incl. dx dy dz) and UMax of the velocity (src/diagnostics.cu:114-130). */
+double or_u2(const or_params *p, const double *uu, const double *detJac);
+double or_umax(const or_params *p, const double *uu);
+
 /* Reductions (src/maths.cu, src/diagnostics.cu:15-82). */
 double or_b2(const or_params *p, const double *B, const double *detJac);   /* incl. dx dy dz */
 double or_jmax(const or_params *p, const double *J);
@@ -135,10 +149,12 @@ typedef struct or_state {
     double *B0, *xb, *B, *detJac, *J, *gradP, *kk, *xb_tmp, *xb_new;
     double t, dt;
     double a[6], b[42], c[6], cs[6];
+    double *uu, *uu_tmp, *uu_new;   /* velocity, 3N inner AoS (inertia only; else NULL) */
 } or_state;
 
 /* Allocates every array and sets B = B0, xb_tmp = xb_new = xb, t = 0,
- * dt = dt0 (src/main.cu:181-256).  B0/xb are copied from the arguments. */
+ * dt = dt0 (src/main.cu:181-256).  B0/xb are copied from the arguments;
+ * with inertia the velocity starts at zero and kk holds 36N values. */
 int  or_state_create(const or_params *p, const double *B0, const double *xb, or_state *s);
 void or_state_free(or_state *s);
 
