@@ -28,6 +28,10 @@ ensemble: 64 seeded braids of the sizes 33x33x97 ... 97x97x289 advanced
 together (one launch per stage for all members); aggregate grid-point
 RK-stages per second.
 
+large: the 257 x 257 x 769 braid (seeded amplitudes, pressure on) run
+for 2000 accepted F32K steps from its initial state; aggregate grid-point
+RK-stages per second over the attempts.
+
 time_to_relax: wall time from the initial state of the 129 x 129 x 385
 braid to max|JxB| / max(|J||B|) <= 0.03, cold and with the learned warm
 start (U-Net inference, up-sampling, checks and 50 settle attempts
@@ -159,6 +163,33 @@ def time_to_relax(g, theta):
             "weights": "glemur_amd/data/unet_braid65x65x193.npz (2048 seeded 65x65x193 braids, 2000 attempts each)"}
 
 
+def large_run(g, workloads, steps):
+    """The 257 x 257 x 769 braid with seeded amplitudes and pressure on, in
+    F32K: `steps` accepted adaptive steps from the initial state, enqueued
+    in batches of 100 attempts with the device controller."""
+    p = workloads.large()
+    n_pts = p.nx * p.ny * p.nz
+    t0 = time.perf_counter()
+    B0, xb = g.init_state(p)
+    ctx = g.Context(p, g.PREC_F32K)
+    ctx.upload(B0, xb, 0.0, p.dt0)
+    del B0, xb
+    ctx.sync()
+    t1 = time.perf_counter()
+    st = ctx.run(max_attempts=10 ** 6, n_cycle=steps, batch=100)
+    t2 = time.perf_counter()
+    log = ctx.step_log(4096)
+    first = log[0] if len(log) == st.n_attempt else None
+    ctx.close()
+    return {"workload": "braid 257x257x769, seeded amplitudes (seed 3), pressure on (beta = 0.1 mean|B0|^2)",
+            "precision": "F32K", "inner_points": n_pts, "accepted": int(st.n_accept),
+            "attempts": int(st.n_attempt), "seconds": t2 - t1, "setup_s": t1 - t0,
+            "value": n_pts * 6 * st.n_attempt / (t2 - t1), "unit": "grid-pt RK-stages/s",
+            "t": st.t, "dt": st.dt,
+            "energy_first": first.energy if first else None, "energy": st.energy,
+            "JxBMax_JBMax_first": first.ratio if first else None, "JxBMax_JBMax": st.ratio}
+
+
 def cpu_baseline(p, B0, xb, budget_s=20.0):
     import oracle as o
     threads = min(16, os.cpu_count() or 1)
@@ -194,6 +225,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0: skip)")
     ap.add_argument("--ttr", type=float, default=0.03,
                     help="time-to-relax threshold for the cold/warm-start comparison (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=2000,
+                    help="accepted steps of the 257x257x769 pressure-on braid (0: skip)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick each stage kernel's tile by timing candidates before measuring")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -274,6 +307,8 @@ def main():
         out["relaxation"] = relaxation_probe(g, p, B0, xb, args.relax_seconds)
     if args.ensemble > 0:
         out["ensemble"] = ensemble(g, workloads, args.ensemble, 20)
+    if args.large_steps > 0:
+        out["large"] = large_run(g, workloads, args.large_steps)
     if args.ttr > 0:
         out["time_to_relax"] = time_to_relax(g, args.ttr)
     if not args.no_cpu:

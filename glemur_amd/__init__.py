@@ -27,6 +27,7 @@ GM_ERR_ARG, GM_ERR_HIP, GM_ERR_IO, GM_ERR_UNSUPPORTED, GM_ERR_NAN, GM_ERR_PARSE 
 PREC_F64, PREC_F32K = 0, 1
 FIELD_XB, FIELD_B0, FIELD_B, FIELD_DETJAC, FIELD_J, FIELD_GRADP, FIELD_KK, FIELD_XB_NEW, FIELD_XB_LO = range(9)
 FIELD_CELLVOL, FIELD_CONVEXITY, FIELD_WEDGEMIN = 9, 10, 11
+FIELD_UU = 12
 
 
 class Params(C.Structure):
@@ -137,6 +138,7 @@ _SIGS = {
     "gm_create": (C.c_int, [_P(Params), C.c_int, _P(_CTX)]),
     "gm_destroy": (C.c_int, [_CTX]),
     "gm_upload": (C.c_int, [_CTX, _D, _D, C.c_double, C.c_double]),
+    "gm_upload_velocity": (C.c_int, [_CTX, _D]),
     "gm_download": (C.c_int, [_CTX, C.c_int, _D]),
     "gm_get_time": (C.c_int, [_CTX, _D, _D]),
     "gm_set_time": (C.c_int, [_CTX, C.c_double, C.c_double]),
@@ -163,8 +165,8 @@ _SIGS = {
     "gm_get_tile": (C.c_int, [_CTX, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "gm_write_b0": (C.c_int, [C.c_char_p, _P(Params), _D, C.c_int]),
     "gm_dump_state": (C.c_int, [C.c_char_p, _P(Params), C.c_double, C.c_double, C.c_int,
-                                 _D, _D, _D, _D, _D, _D, _D, C.c_int]),
-    "gm_read_state": (C.c_int, [C.c_char_p, _P(Params), _D, _D, _D, _D, C.c_int]),
+                                _D, _D, _D, _D, _D, _D, _D, _D, C.c_int]),
+    "gm_read_state": (C.c_int, [C.c_char_p, _P(Params), _D, _D, _D, _D, _D, C.c_int]),
     "gm_write_ts": (C.c_int, [C.c_char_p, _P(Params), _P(Reduction), C.c_double, C.c_double, C.c_int,
                                C.c_double, C.c_int, C.c_int]),
     "gm_write_device_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int]),
@@ -303,11 +305,18 @@ class Context:
         assert B0.size == 3 * self.p.npad and xb.size == 3 * self.p.npad
         _check(_lib.gm_upload(self._h, _dptr(B0), _dptr(xb), t, self.p.dt0 if dt is None else dt))
 
+    def upload_velocity(self, uu):
+        """Velocity of the inner points (inner AoS, 3N); inertia only."""
+        uu = np.ascontiguousarray(uu, dtype=np.float64)
+        assert uu.size == 3 * self.p.ninner
+        _check(_lib.gm_upload_velocity(self._h, _dptr(uu)))
+
     def download(self, field):
         sizes = {FIELD_XB: 3 * self.p.npad, FIELD_B0: 3 * self.p.npad, FIELD_B: 3 * self.p.npad,
                  FIELD_DETJAC: self.p.npad, FIELD_J: 3 * self.p.ninner, FIELD_GRADP: 3 * self.p.ninner,
-                 FIELD_KK: 18 * self.p.ninner, FIELD_XB_NEW: 3 * self.p.npad, FIELD_XB_LO: 3 * self.p.npad,
-                 FIELD_CELLVOL: self.p.ninner, FIELD_CONVEXITY: self.p.ninner, FIELD_WEDGEMIN: self.p.ninner}
+                 FIELD_KK: (36 if self.p.inertia else 18) * self.p.ninner, FIELD_XB_NEW: 3 * self.p.npad,
+                 FIELD_XB_LO: 3 * self.p.npad, FIELD_CELLVOL: self.p.ninner, FIELD_CONVEXITY: self.p.ninner,
+                 FIELD_WEDGEMIN: self.p.ninner, FIELD_UU: 3 * self.p.ninner}
         out = np.empty(sizes[field])
         _check(_lib.gm_download(self._h, field, _dptr(out)))
         return out
@@ -448,19 +457,25 @@ def write_b0(dirpath, p, B0, precision_bits=64):
 
 
 def dump_state(dirpath, p, t, dt, n, xb, B, J=None, detJac=None, precision_bits=64, cellVol=None,
-               convexity=None, wedgeMin=None):
+               convexity=None, wedgeMin=None, uu=None):
     nul = C.cast(None, _D)
     opt = lambda a: _dptr(a) if a is not None else nul
-    _check(_lib.gm_dump_state(str(dirpath).encode(), C.byref(p), t, dt, n, _dptr(xb), _dptr(B), opt(J), opt(detJac),
-                              opt(cellVol), opt(convexity), opt(wedgeMin), precision_bits))
+    _check(_lib.gm_dump_state(str(dirpath).encode(), C.byref(p), t, dt, n, _dptr(xb), _dptr(B), opt(uu), opt(J),
+                              opt(detJac), opt(cellVol), opt(convexity), opt(wedgeMin), precision_bits))
 
 
 def read_state(dirpath, p, precision_bits=64):
+    """(xb, B0, t, dt) of a restart, and the velocity (inner AoS) as a
+    fifth item when p.inertia."""
     xb = np.empty(3 * p.npad)
     B0 = np.empty(3 * p.npad)
+    uu = np.empty(3 * p.ninner) if p.inertia else None
     t, dt = C.c_double(), C.c_double()
-    _check(_lib.gm_read_state(str(dirpath).encode(), C.byref(p), _dptr(xb), _dptr(B0), C.byref(t),
-                              C.byref(dt), precision_bits))
+    _check(_lib.gm_read_state(str(dirpath).encode(), C.byref(p), _dptr(xb), _dptr(B0),
+                              _dptr(uu) if uu is not None else C.cast(None, _D), C.byref(t), C.byref(dt),
+                              precision_bits))
+    if p.inertia:
+        return xb, B0, t.value, dt.value, uu
     return xb, B0, t.value, dt.value
 
 

@@ -20,6 +20,11 @@ struct gm_ctx {
     // F64
     double *X64[3] = {nullptr, nullptr, nullptr};
     double *B0_64 = nullptr, *K64 = nullptr, *B64 = nullptr, *J64 = nullptr, *G64 = nullptr, *Xtmp64 = nullptr;
+    // F64 with inertia: velocity (padded SoA, inner points used; U64[ucur]
+    // current, the other the candidate of the last attempt), its stage copy
+    // and six velocity slopes
+    double *U64[2] = {nullptr, nullptr}, *Utmp64 = nullptr, *KU64 = nullptr;
+    int ucur = 0;
     // F32K
     float *Xh[2] = {nullptr, nullptr}, *Xl[2] = {nullptr, nullptr};
     float *B0_32 = nullptr, *K32 = nullptr, *B32 = nullptr, *J32 = nullptr, *G32 = nullptr;

@@ -20,6 +20,8 @@ struct Geo {
     double beta, maxError, dtMin;
     int jxbAver;              // J x B averaged with the face neighbours (per-kernel path)
     double jxbAverWeight;
+    int inertia;              // velocity equation with friction nu (per-kernel path)
+    double nu;
     DistParams dist;
 };
 
@@ -58,11 +60,21 @@ void launch_jstokes(hipStream_t s, const Geo &g, const double *X, const double *
 void launch_jclassic(hipStream_t s, const Geo &g, const double *X, const double *B0, const double *det, double *J);
 void launch_jstokes4th(hipStream_t s, const Geo &g, const double *X, const double *B, double *J);
 void launch_gradp(hipStream_t s, const Geo &g, const double *X, const double *det, double *G);
+// With g.inertia, K holds the position slopes dt u, KU the velocity slopes
+// and U / Utmp the stage velocity (padded SoA, like the positions); det is
+// read only then.  Without inertia U, KU, Utmp and det may be null.
 void launch_kk(hipStream_t s, const Geo &g, const Tableau *tab, int n, double dt, const double *X,
-               const double *B, const double *J, const double *G, double *K, double *Xtmp);
-// x_new and per-block error maxima; returns the number of partials written.
+               const double *B, const double *J, const double *G, double *K, double *Xtmp, const double *det,
+               const double *U, double *KU, double *Utmp);
+// x_new (and u_new with inertia) and per-block error maxima of the
+// positions; returns the number of partials written.
 int launch_xnewstar(hipStream_t s, const Geo &g, const Tableau *tab, const double *X, const double *K,
-                    double *Xnew, double *partials, int *nanflag);
+                    double *Xnew, double *partials, int *nanflag, const double *U, const double *KU, double *Unew);
+// U2 (incl. dx dy dz, boundary weights 1/2^faces) into out[0] and the
+// NaN-blind max |u| into out[1]; part holds 2 * 512 doubles.
+void launch_u_diag(hipStream_t s, const Geo &g, const double *U, const double *det, double *part, double *out);
+// dst (padded SoA, inner points) <- src (inner AoS 3N)
+void launch_inner_aos_to_padded(hipStream_t s, const Geo &g, const double *src, double *dst);
 // Final NaN-blind max of n partials into out[0].
 void launch_max_final(hipStream_t s, const double *partials, int n, double *out);
 // Diagnostics over inner points from B, J (padded SoA) and det:

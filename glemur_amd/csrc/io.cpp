@@ -101,8 +101,9 @@ extern "C" int gm_write_b0(const char *dir, const gm_params *p, const double *B0
 
 // dumpState + writeParams, src/io.cu:209-386, 503-538
 extern "C" int gm_dump_state(const char *dir, const gm_params *p, double t, double dt, int n,
-                             const double *xb, const double *B, const double *J, const double *detJac,
-                             const double *cellVol, const double *convexity, const double *wedgeMin, int precision)
+                             const double *xb, const double *B, const double *uu, const double *J,
+                             const double *detJac, const double *cellVol, const double *convexity,
+                             const double *wedgeMin, int precision)
 {
     if (!p || !xb || !B) return fail(GM_ERR_ARG, "gm_dump_state: null pointer");
     bool dbl = precision != 32;
@@ -156,18 +157,27 @@ extern "C" int gm_dump_state(const char *dir, const gm_params *p, double t, doub
     fprintf(fd, "POINT_DATA %9lld\n", P);
     fprintf(fd, "VECTORS bfield %s\n", real_str(dbl));
     vec_block(B);
-    if (p->dumpJ && J) {
-        // J lives on inner points; ghosts are written as zeros (src/io.cu:295-317)
-        fprintf(fd, "VECTORS jfield %s\n", real_str(dbl));
+    // the velocity lives on inner points; ghosts are written as zeros
+    // (src/io.cu:269-290)
+    auto inner_vec = [&](const double *v) {
         for (int k = 0; k < nz + 2; ++k)
             for (int j = 0; j < ny + 2; ++j)
                 for (int i = 0; i < nx + 2; ++i) {
                     bool ghost = i == 0 || i == nx + 1 || j == 0 || j == ny + 1 || k == 0 || k == nz + 1;
                     for (int l = 0; l < 3; ++l)
-                        be.real(ghost ? 0. : J[l + 3 * ((i - 1) + (long long)nx * ((j - 1) + (long long)ny * (k - 1)))], dbl);
+                        be.real(ghost ? 0. : v[l + 3 * ((i - 1) + (long long)nx * ((j - 1) + (long long)ny * (k - 1)))], dbl);
                     if (be.buf.size() > (1 << 20)) ok &= be.flush(fd);
                 }
         ok &= be.flush(fd);
+    };
+    if (p->inertia && uu) {
+        fprintf(fd, "VECTORS ufield %s\n", real_str(dbl));
+        inner_vec(uu);
+    }
+    if (p->dumpJ && J) {
+        // J lives on inner points; ghosts are written as zeros (src/io.cu:295-317)
+        fprintf(fd, "VECTORS jfield %s\n", real_str(dbl));
+        inner_vec(J);
     }
     if (p->dumpDetJac && detJac) {
         fprintf(fd, "SCALARS detJac %s\n", real_str(dbl));
@@ -201,11 +211,12 @@ extern "C" int gm_dump_state(const char *dir, const gm_params *p, double t, doub
 // readState, src/io.cu:390-499.  FIELD entries are skipped by their declared
 // size, so every entry, maxError included, is consumed (the reference stops
 // reading inside the maxError bytes, src/io.cu:415-460).
-extern "C" int gm_read_state(const char *dir, const gm_params *p, double *xb, double *B0, double *t,
+extern "C" int gm_read_state(const char *dir, const gm_params *p, double *xb, double *B0, double *uu, double *t,
                              double *dt, int precision)
 {
     (void)precision;
     if (!p || !xb || !B0 || !t || !dt) return fail(GM_ERR_ARG, "gm_read_state: null pointer");
+    if (p->inertia && !uu) return fail(GM_ERR_ARG, "gm_read_state: inertia = t needs the velocity array");
     const long long P = gm::npad(*p);
     std::string fn = path_in(dir, "data/save.vtk");
     FILE *fd = fopen(fn.c_str(), "rb");
@@ -245,6 +256,25 @@ extern "C" int gm_read_state(const char *dir, const gm_params *p, double *xb, do
     std::vector<unsigned char> b((size_t)3 * P * sz);
     if (fread(b.data(), 1, b.size(), fd) != b.size()) { fclose(fd); return fail(GM_ERR_IO, fn + ": truncated POINTS"); }
     for (long long a = 0; a < 3 * P; ++a) xb[a] = rd_be(b.data() + a * sz, sz == 8);
+    if (p->inertia) {
+        // the velocity follows B (src/io.cu:466-479).  The blocks are found
+        // by their headers: the reference reads B from the start of the
+        // "VECTORS bfield" line, one line early, and the velocity shifted by
+        // as much; that misalignment is not reproduced
+        if (!getline() || strncmp(line, "POINT_DATA", 10) != 0) { fclose(fd); return fail(GM_ERR_IO, fn + ": no POINT_DATA"); }
+        if (!getline() || strncmp(line, "VECTORS bfield", 14) != 0) { fclose(fd); return fail(GM_ERR_IO, fn + ": no bfield"); }
+        if (fread(b.data(), 1, b.size(), fd) != b.size()) { fclose(fd); return fail(GM_ERR_IO, fn + ": truncated bfield"); }
+        if (!getline() || strncmp(line, "VECTORS ufield", 14) != 0) { fclose(fd); return fail(GM_ERR_IO, fn + ": no ufield"); }
+        if (fread(b.data(), 1, b.size(), fd) != b.size()) { fclose(fd); return fail(GM_ERR_IO, fn + ": truncated ufield"); }
+        const int nx = p->nx, ny = p->ny, nz = p->nz;
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const long long a = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (k + 1));
+                    const long long q = i + (long long)nx * (j + (long long)ny * k);
+                    for (int l = 0; l < 3; ++l) uu[3 * q + l] = rd_be(b.data() + (3 * a + l) * sz, sz == 8);
+                }
+    }
     fclose(fd);
 
     std::string fb = path_in(dir, "data/B0.vtk");

@@ -436,7 +436,8 @@ __global__ void __launch_bounds__(kBlock) k_gradp(Geo g, const double *X, const 
 // --------------------------------------------------------------------- kk
 __global__ void __launch_bounds__(kBlock) k_kk(Geo g, const Tableau *tab, int n, double dt, const double *X,
                                                const double *B, const double *J, const double *G, double *K,
-                                               double *Xtmp)
+                                               double *Xtmp, const double *det, const double *U, double *KU,
+                                               double *Utmp)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long N = (long long)g.nx * g.ny * g.nz;
@@ -448,6 +449,31 @@ __global__ void __launch_bounds__(kBlock) k_kk(Geo g, const Tableau *tab, int n,
     ld3(J, g.P, a, jv);
     ld3(B, g.P, a, bv);
     cross3(jv, bv, kv);
+    if (g.inertia) {
+        // inertia branch (src/rungeKutta.cu:195-212, 216-246): position
+        // slope dt u, velocity slope dt det (J x B) - dt nu u det - dt gradP;
+        // no friction profile and no averaging here
+        const double dj = det[a];
+        for (int l = 0; l < 3; ++l) {
+            const double u = U[l * g.P + a];
+            K[n * P3 + l * g.P + a] = dt * u;
+            kv[l] *= dt * dj;
+            kv[l] -= dt * g.nu * u * dj;
+            if (g.pressure) kv[l] += -dt * G[l * g.P + a];
+            KU[n * P3 + l * g.P + a] = kv[l];
+        }
+        for (int l = 0; l < 3; ++l) {
+            double x = X[l * g.P + a], v = U[l * g.P + a];
+            for (int m = 0; m < n + 1; ++m) {
+                const double w = tab->b[m + (n + 1) * 6] - tab->b[m + n * 6];
+                x += w * K[m * P3 + l * g.P + a];
+                v += w * KU[m * P3 + l * g.P + a];
+            }
+            Xtmp[l * g.P + a] = x;
+            Utmp[l * g.P + a] = v;
+        }
+        return;
+    }
     if (g.jxbAver) {
         // jxbAver (src/rungeKutta.cu:172-177, 357-443): (1 - w) J x B plus
         // w/6 J x B of each face neighbour that is an inner point, in the
@@ -486,7 +512,8 @@ __global__ void __launch_bounds__(kBlock) k_kk(Geo g, const Tableau *tab, int n,
 
 // --------------------------------------------------------------- xNewStar
 __global__ void __launch_bounds__(kBlock) k_xnewstar(Geo g, const Tableau *tab, const double *X, const double *K,
-                                                     double *Xnew, double *partials, int *nanflag)
+                                                     double *Xnew, double *partials, int *nanflag, const double *U,
+                                                     const double *KU, double *Unew)
 {
     __shared__ double wmax[kBlock / 64];
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -510,6 +537,12 @@ __global__ void __launch_bounds__(kBlock) k_xnewstar(Geo g, const Tableau *tab, 
             if (d != d) nan = true;
             m = nb_max(m, d);
         }
+        if (g.inertia)   // u5 (src/rungeKutta.cu:311-322); no error term
+            for (int l = 0; l < 3; ++l) {
+                double un = U[l * g.P + a];
+                for (int s = 0; s < 6; ++s) un += tab->c[s] * KU[s * P3 + l * g.P + a];
+                Unew[l * g.P + a] = un;
+            }
         if (nan) atomicOr(nanflag, 1);
     }
     m = wave_max(m);
@@ -535,6 +568,66 @@ __global__ void __launch_bounds__(1024) k_max_final(const double *partials, int 
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = nb_max(r, wmax[w]);
         out[0] = r;
     }
+}
+
+// ------------------------------------------------------- velocity diagnostics
+// U2_det (src/maths.cu:683-717): |u|^2 det / 2^(#boundary faces) summed,
+// and mag + findDevMax (src/maths.cu:157-164, 58-130): NaN-blind max |u|.
+// 512 grid-stride blocks write partials; one thread combines them in order.
+constexpr int kUBlocks = 512;
+
+__global__ void __launch_bounds__(kBlock) k_u_diag(Geo g, const double *U, const double *det, double *part)
+{
+    __shared__ double ss[kBlock], sm[kBlock];
+    const long long N = (long long)g.nx * g.ny * g.nz;
+    double e = 0, m = 0;
+    for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < N; t += (long long)gridDim.x * kBlock) {
+        const int i = (int)(t % g.nx), j = (int)((t / g.nx) % g.ny), k = (int)(t / ((long long)g.nx * g.ny));
+        const long long a = gidx(g, i + 1, j + 1, k + 1);
+        const double u0 = U[a], u1 = U[g.P + a], u2 = U[2 * g.P + a];
+        double v = (u0 * u0 + u1 * u1 + u2 * u2) * det[a];
+        int bound = (i == 0) + (j == 0) + (k == 0) + (i == g.nx - 1) + (j == g.ny - 1) + (k == g.nz - 1);
+        bound = 1 << bound;
+        v /= bound;
+        e += v;
+        m = nb_max(m, sqrt(u0 * u0 + u1 * u1 + u2 * u2));
+    }
+    ss[threadIdx.x] = e;
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            ss[threadIdx.x] += ss[threadIdx.x + s];
+            sm[threadIdx.x] = nb_max(sm[threadIdx.x], sm[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ss[0];
+        part[kUBlocks + blockIdx.x] = sm[0];
+    }
+}
+
+__global__ void k_u_final(Geo g, const double *part, double *out)
+{
+    if (threadIdx.x != 0) return;
+    double e = 0, m = 0;
+    for (int b = 0; b < kUBlocks; ++b) {
+        e += part[b];
+        m = nb_max(m, part[kUBlocks + b]);
+    }
+    out[0] = e * g.dx * g.dy * g.dz;
+    out[1] = m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_inner_aos_to_padded(Geo g, const double *src, double *dst)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long N = (long long)g.nx * g.ny * g.nz;
+    if (t >= N) return;
+    const int i = (int)(t % g.nx), j = (int)((t / g.nx) % g.ny), k = (int)(t / ((long long)g.nx * g.ny));
+    const long long a = gidx(g, i + 1, j + 1, k + 1);
+    for (int l = 0; l < 3; ++l) dst[l * g.P + a] = src[3 * t + l];
 }
 
 // ------------------------------------------------------------ diagnostics
@@ -726,16 +819,27 @@ void launch_gradp(hipStream_t s, const Geo &g, const double *X, const double *de
     hipLaunchKernelGGL(k_gradp, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, X, det, G);
 }
 void launch_kk(hipStream_t s, const Geo &g, const Tableau *tab, int n, double dt, const double *X, const double *B,
-               const double *J, const double *G, double *K, double *Xtmp)
+               const double *J, const double *G, double *K, double *Xtmp, const double *det, const double *U,
+               double *KU, double *Utmp)
 {
-    hipLaunchKernelGGL(k_kk, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, tab, n, dt, X, B, J, G, K, Xtmp);
+    hipLaunchKernelGGL(k_kk, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, tab, n, dt, X, B, J, G, K, Xtmp, det,
+                       U, KU, Utmp);
 }
 int launch_xnewstar(hipStream_t s, const Geo &g, const Tableau *tab, const double *X, const double *K, double *Xnew,
-                    double *partials, int *nanflag)
+                    double *partials, int *nanflag, const double *U, const double *KU, double *Unew)
 {
     int nb = blocks_for(ninner(g));
-    hipLaunchKernelGGL(k_xnewstar, dim3(nb), dim3(kBlock), 0, s, g, tab, X, K, Xnew, partials, nanflag);
+    hipLaunchKernelGGL(k_xnewstar, dim3(nb), dim3(kBlock), 0, s, g, tab, X, K, Xnew, partials, nanflag, U, KU, Unew);
     return nb;
+}
+void launch_u_diag(hipStream_t s, const Geo &g, const double *U, const double *det, double *part, double *out)
+{
+    hipLaunchKernelGGL(k_u_diag, dim3(kUBlocks), dim3(kBlock), 0, s, g, U, det, part);
+    hipLaunchKernelGGL(k_u_final, dim3(1), dim3(64), 0, s, g, part, out);
+}
+void launch_inner_aos_to_padded(hipStream_t s, const Geo &g, const double *src, double *dst)
+{
+    hipLaunchKernelGGL(k_inner_aos_to_padded, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, src, dst);
 }
 void launch_max_final(hipStream_t s, const double *partials, int n, double *out)
 {

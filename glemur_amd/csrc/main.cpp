@@ -78,6 +78,7 @@ int main(int argc, char *argv[])
     const long long P3 = 3LL * (p.nx + 2) * (p.ny + 2) * (p.nz + 2);
     const long long N3 = 3LL * p.nx * p.ny * p.nz;
     std::vector<double> B0(P3), xb(P3), B(P3), J(N3), det(P3 / 3), cvol(N3 / 3), conv(N3 / 3), wmin(N3 / 3);
+    std::vector<double> uu(p.inertia ? N3 : 0);   // inner AoS; zero unless restarted
     double t = 0, dt = p.dt0;
     gm_reduction red{};
     red.B2res = gm_init_residual(&p);
@@ -89,12 +90,14 @@ int main(int argc, char *argv[])
         t = 0;
         gm_write_ts(".", &p, &red, t, dt, 0, 0., 1, 1);
     } else {
-        if (gm_read_state(".", &p, xb.data(), B0.data(), &t, &dt, 64) != GM_OK) return die("restart");
+        if (gm_read_state(".", &p, xb.data(), B0.data(), p.inertia ? uu.data() : nullptr, &t, &dt, 64) != GM_OK)
+            return die("restart");
     }
 
     gm_ctx *ctx = nullptr;
     if (gm_create(&p, prec_bits == 64 ? GM_PREC_F64 : GM_PREC_F32K, &ctx) != GM_OK) return die("context");
     if (gm_upload(ctx, B0.data(), xb.data(), t, dt) != GM_OK) return die("upload");
+    if (p.inertia && p.fRestart && gm_upload_velocity(ctx, uu.data()) != GM_OK) return die("upload velocity");
 
     // time stepping, src/main.cu:259-384
     int timeIndex;
@@ -125,13 +128,15 @@ int main(int argc, char *argv[])
                     return die("download");
                 if ((p.dumpCellVol && gm_download(ctx, GM_FIELD_CELLVOL, cvol.data())) ||
                     (p.dumpConvexity && gm_download(ctx, GM_FIELD_CONVEXITY, conv.data())) ||
-                    (p.dumpWedgeMin && gm_download(ctx, GM_FIELD_WEDGEMIN, wmin.data())))
+                    (p.dumpWedgeMin && gm_download(ctx, GM_FIELD_WEDGEMIN, wmin.data())) ||
+                    (p.inertia && gm_download(ctx, GM_FIELD_UU, uu.data())))
                     return die("download");
+                const double *u = p.inertia ? uu.data() : nullptr;
                 if (dump)
-                    gm_dump_state(".", &p, t, dt, (int)ceil((t - dt) / p.dtDump), xb.data(), B.data(), J.data(),
+                    gm_dump_state(".", &p, t, dt, (int)ceil((t - dt) / p.dtDump), xb.data(), B.data(), u, J.data(),
                                   det.data(), cvol.data(), conv.data(), wmin.data(), 64);
                 if (save)
-                    gm_dump_state(".", &p, t, dt, -1, xb.data(), B.data(), J.data(), det.data(), cvol.data(),
+                    gm_dump_state(".", &p, t, dt, -1, xb.data(), B.data(), u, J.data(), det.data(), cvol.data(),
                                   conv.data(), wmin.data(), 64);
             }
         } else {

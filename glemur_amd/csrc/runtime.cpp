@@ -58,6 +58,7 @@ gmd::Geo make_geo(const gm_params &p)
     g.epsilonProf = p.epsilonProf; g.zUpdate = p.zUpdate; g.pressure = p.pressure;
     g.beta = p.beta; g.maxError = p.maxError; g.dtMin = p.dtMin;
     g.jxbAver = p.jxbAver; g.jxbAverWeight = p.jxbAverWeight;
+    g.inertia = p.inertia; g.nu = p.nu;
     g.dist.code = p.initDistCode;
     g.dist.shearA = p.initShearA; g.dist.shearB = p.initShearB; g.dist.shearK = p.initShearK;
     g.dist.dx = p.dx; g.dist.dy = p.dy; g.dist.Ox = p.Ox; g.dist.Oy = p.Oy;
@@ -285,7 +286,8 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     if (!pp || !out) return fail(GM_ERR_ARG, "gm_create: null pointer");
     if (precision != GM_PREC_F64 && precision != GM_PREC_F32K) return fail(GM_ERR_ARG, "unknown precision");
     if (pp->nx < 2 || pp->ny < 2 || pp->nz < 2) return fail(GM_ERR_ARG, "grid needs at least 2 inner points per axis");
-    if (pp->inertia) return fail(GM_ERR_UNSUPPORTED, "inertia = t is not supported (magnetofrictional equation only)");
+    if (pp->inertia && precision != GM_PREC_F64)
+        return fail(GM_ERR_UNSUPPORTED, "inertia = t runs on the FP64 per-kernel path only");
     if (pp->jxbAver && precision != GM_PREC_F64)
         return fail(GM_ERR_UNSUPPORTED, "jxbAver = t runs on the FP64 per-kernel path only");
     const bool classic = gm::starts(pp->jMethod, "Classic "), stokes4 = gm::starts(pp->jMethod, "Stokes4th ");
@@ -330,6 +332,13 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
         (void)hipMemset(c->K64, 0, 6 * P3 * sizeof(double));
         (void)hipMemset(c->J64, 0, P3 * sizeof(double));
         (void)hipMemset(c->G64, 0, P3 * sizeof(double));
+        if (pp->inertia) {
+            if ((r = dalloc(&c->U64[0], P3)) || (r = dalloc(&c->U64[1], P3)) || (r = dalloc(&c->Utmp64, P3)) ||
+                (r = dalloc(&c->KU64, 6 * P3)))
+                return bail(r);
+            for (double *u : {c->U64[0], c->U64[1], c->Utmp64}) (void)hipMemset(u, 0, P3 * sizeof(double));
+            (void)hipMemset(c->KU64, 0, 6 * P3 * sizeof(double));
+        }
     } else {
         for (int i = 0; i < 2; ++i)
             if ((r = dalloc(&c->Xh[i], P3)) || (r = dalloc(&c->Xl[i], P3))) return bail(r);
@@ -398,6 +407,7 @@ extern "C" int gm_destroy(gm_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (int i = 0; i < 3; ++i) dfree(c->X64[i]);
     dfree(c->B0_64); dfree(c->K64); dfree(c->B64); dfree(c->J64); dfree(c->G64); dfree(c->Xtmp64);
+    dfree(c->U64[0]); dfree(c->U64[1]); dfree(c->Utmp64); dfree(c->KU64);
     for (int i = 0; i < 2; ++i) { dfree(c->Xh[i]); dfree(c->Xl[i]); }
     dfree(c->B0_32); dfree(c->K32); dfree(c->B32); dfree(c->J32); dfree(c->G32);
     dfree(c->det); dfree(c->epsx); dfree(c->epsy); dfree(c->epsz);
@@ -434,6 +444,12 @@ extern "C" int gm_upload(gm_ctx *c, const double *B0, const double *xb, double t
         gmd::launch_aos_to_soa_f32(s, c->P, stage, c->B0_32);
         HIPCHK(hipMemcpyAsync(c->B32, c->B0_32, P3 * 4, hipMemcpyDeviceToDevice, s));
     }
+    if (c->p.inertia) {
+        // the velocity starts at rest (uInit = 'nil'); gm_upload_velocity
+        // sets another one
+        for (int i = 0; i < 2; ++i) gmd::launch_fill_f64(s, c->U64[i], (long long)P3, 0.0);
+        c->ucur = 0;
+    }
     HIPCHK(hipStreamSynchronize(s));
     dfree(stage);
     gmd::Ctl &h = *c->ctlh;
@@ -450,6 +466,23 @@ extern "C" int gm_upload(gm_ctx *c, const double *B0, const double *xb, double t
     gmd::FusedArgs A = fused_args(c, 0, 0, 0, 0);
     gmd::launch_ghost_setup(c->stream, A, c->prec == GM_PREC_F32K, c->ghost);
     HIPCHK(hipStreamSynchronize(c->stream));
+    return GM_OK;
+}
+
+// cudaMemcpy(d.uu, h.uu), src/main.cu:216-222: the velocity of inner
+// points, inner AoS (the reference's h.uu layout)
+extern "C" int gm_upload_velocity(gm_ctx *c, const double *uu)
+{
+    if (!c || !uu) return fail(GM_ERR_ARG, "gm_upload_velocity: null pointer");
+    if (!c->p.inertia) return fail(GM_ERR_ARG, "gm_upload_velocity: the context was created with inertia = f");
+    const size_t N3 = 3 * (size_t)c->N;
+    double *stage = nullptr;
+    int r = dalloc(&stage, N3);
+    if (r) return r;
+    HIPCHK(hipMemcpyAsync(stage, uu, N3 * 8, hipMemcpyHostToDevice, c->stream));
+    gmd::launch_inner_aos_to_padded(c->stream, c->geo, stage, c->U64[c->ucur]);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(stage);
     return GM_OK;
 }
 
@@ -539,7 +572,36 @@ extern "C" int gm_download(gm_ctx *c, int field, double *host)
         if (r) { dfree(tmp); return r; }
         break;
     }
+    case GM_FIELD_UU: {
+        if (!c->p.inertia) return fail(GM_ERR_ARG, "gm_download: no velocity without inertia");
+        n = N3;
+        if ((r = dalloc(&tmp, n))) return r;
+        gmd::launch_padded_to_inner_aos<double>(s, c->geo, c->U64[c->ucur], tmp);
+        break;
+    }
     case GM_FIELD_KK: {
+        if (c->p.inertia) {
+            // [stage][point][x0 x1 x2 u0 u1 u2], the reference's nVar = 6 layout
+            n = 12 * N3;
+            if ((r = dalloc(&tmp, n))) return r;
+            for (int m = 0; m < 6; ++m) {
+                gmd::launch_padded_to_inner_aos<double>(s, c->geo, c->K64 + m * P3, tmp + m * N3);
+                gmd::launch_padded_to_inner_aos<double>(s, c->geo, c->KU64 + m * P3, tmp + (6 + m) * N3);
+            }
+            std::vector<double> h(n);
+            hipError_t e = hipMemcpyAsync(h.data(), tmp, n * 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            dfree(tmp);
+            if (e != hipSuccess) return fail(GM_ERR_HIP, std::string("download: ") + hipGetErrorString(e));
+            const size_t N = (size_t)c->N;
+            for (int m = 0; m < 6; ++m)
+                for (size_t q = 0; q < N; ++q)
+                    for (int l = 0; l < 3; ++l) {
+                        host[6 * N * m + 6 * q + l] = h[N3 * m + 3 * q + l];
+                        host[6 * N * m + 6 * q + 3 + l] = h[N3 * (6 + m) + 3 * q + l];
+                    }
+            return GM_OK;
+        }
         n = 6 * N3;
         if ((r = dalloc(&tmp, n))) return r;
         (void)hipMemsetAsync(tmp, 0, n * 8, s);
@@ -620,9 +682,14 @@ extern "C" int gm_rk_kk(gm_ctx *c, int n, double dt)
     }
     // xb_tmp starts as xb so its ghost layers match (src/main.cu:202)
     HIPCHK(hipMemcpyAsync(c->Xtmp64, c->X64[c->cur], P3 * 8, hipMemcpyDeviceToDevice, c->stream));
-    gmd::launch_kk(c->stream, c->geo, c->tab_dev, n, dt, c->X64[c->cur], c->B64, c->J64, c->G64, c->K64, c->Xtmp64);
+    const bool in = c->p.inertia;
+    gmd::launch_kk(c->stream, c->geo, c->tab_dev, n, dt, c->X64[c->cur], c->B64, c->J64, c->G64, c->K64, c->Xtmp64,
+                   in ? c->det : nullptr, in ? c->U64[c->ucur] : nullptr, in ? c->KU64 : nullptr,
+                   in ? c->Utmp64 : nullptr);
     if (any_peri(c->p)) gmd::launch_set_grid_peri(c->stream, c->geo, c->Xtmp64);
     HIPCHK(hipMemcpyAsync(c->X64[c->cur], c->Xtmp64, P3 * 8, hipMemcpyDeviceToDevice, c->stream));
+    // uu <- uu_tmp (src/main.cu:296-305)
+    if (in) HIPCHK(hipMemcpyAsync(c->U64[c->ucur], c->Utmp64, P3 * 8, hipMemcpyDeviceToDevice, c->stream));
     return GM_OK;
 }
 extern "C" int gm_rk_xnewstar(gm_ctx *c, double *maxDelta)
@@ -637,7 +704,10 @@ extern "C" int gm_rk_xnewstar(gm_ctx *c, double *maxDelta)
     double *xnew = c->X64[(c->cur + 1) % 3];
     // xb_new starts as xb (src/main.cu:209) so its ghost layers match
     HIPCHK(hipMemcpyAsync(xnew, c->X64[c->cur], 3 * (size_t)c->P * 8, hipMemcpyDeviceToDevice, c->stream));
-    int n = gmd::launch_xnewstar(c->stream, c->geo, c->tab_dev, c->X64[c->cur], c->K64, xnew, parts, c->nanflag);
+    const bool in = c->p.inertia;
+    int n = gmd::launch_xnewstar(c->stream, c->geo, c->tab_dev, c->X64[c->cur], c->K64, xnew, parts, c->nanflag,
+                                 in ? c->U64[c->ucur] : nullptr, in ? c->KU64 : nullptr,
+                                 in ? c->U64[c->ucur ^ 1] : nullptr);
     if (any_peri(c->p)) gmd::launch_set_grid_peri(c->stream, c->geo, xnew);
     gmd::launch_max_final(c->stream, parts, n, c->red_out);
     HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 8, hipMemcpyDeviceToHost, c->stream));
@@ -653,6 +723,7 @@ extern "C" int gm_accept(gm_ctx *c)
     if (r) return r;
     c->ctlh->cur = c->prec == GM_PREC_F64 ? (c->ctlh->cur + 1) % 3 : c->ctlh->cur ^ 1;
     c->cur = c->ctlh->cur;
+    if (c->p.inertia) c->ucur ^= 1;   // uu <- uu_new (src/main.cu:331-338)
     return push_ctl(c);
 }
 
@@ -685,9 +756,15 @@ extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
     if (mesh || errs || eps) {
         if (int r = extra_diag(c, mesh, errs, eps, nullptr, nullptr, nullptr)) return r;
     }
+    const bool udiag = p.inertia && (p.redU2 || p.redUMax);
+    if (udiag) gmd::launch_u_diag(c->stream, c->geo, c->U64[c->ucur], c->det, c->diag_scratch, c->red_out + 13);
     HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 16 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     memset(red, 0, sizeof *red);
+    if (udiag) {
+        red->U2 = c->red_host[13];
+        red->UMax = c->red_host[14];
+    }
     red->B2 = c->red_host[0];
     red->JMax = c->red_host[1];
     red->JxB_B2Max = c->red_host[2];
@@ -714,15 +791,17 @@ static int check_fused(gm_ctx *c)
     if (!gm::starts(c->p.jMethod, "Stokes "))
         return fail(GM_ERR_UNSUPPORTED, "jMethod Classic / Stokes4th run through the per-kernel path (gm_current ...)");
     if (c->p.jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver runs through the per-kernel path (gm_rk_kk ...)");
+    if (c->p.inertia) return fail(GM_ERR_UNSUPPORTED, "inertia runs through the per-kernel path (gm_rk_kk ...)");
     return GM_OK;
 }
 
 // FP64 contexts the fused stage kernels do not cover (periodic faces,
-// jMethod = Classic or Stokes4th, jxbAver) step through the per-kernel
-// sequence
+// jMethod = Classic or Stokes4th, jxbAver, inertia) step through the
+// per-kernel sequence
 static bool per_kernel_path(gm_ctx *c)
 {
-    return c->prec == GM_PREC_F64 && (any_peri(c->p) || !gm::starts(c->p.jMethod, "Stokes ") || c->p.jxbAver);
+    return c->prec == GM_PREC_F64 &&
+           (any_peri(c->p) || !gm::starts(c->p.jMethod, "Stokes ") || c->p.jxbAver || c->p.inertia);
 }
 
 // Fields of the current state through the per-kernel sequence
@@ -828,6 +907,7 @@ static int step_per_kernel(gm_ctx *c)
     if (acc) {
         h.t += dt;
         h.cur = (h.cur + 1) % 3;
+        if (c->p.inertia) c->ucur ^= 1;
         h.n_accept += 1;
     }
     if (md > 0) h.dt = dt * std::pow(0.9 * c->p.maxError / md, 0.2);

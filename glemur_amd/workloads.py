@@ -81,8 +81,12 @@ def mean_b0_sq(p):
     """Mean of |B0|^2 over inner points of the blobs field, evaluated on the
     host from the closed form."""
     B0, _ = init_state(p)
-    b = B0.reshape(p.nz + 2, p.ny + 2, p.nx + 2, 3)[1:-1, 1:-1, 1:-1]
-    return float(np.mean(np.sum(b * b, axis=-1)))
+    b = B0.reshape(p.nz + 2, p.ny + 2, p.nx + 2, 3)
+    s = 0.0
+    for k in range(1, p.nz + 1):          # plane by plane: no full-size temporaries
+        q = b[k, 1:-1, 1:-1]
+        s += float(np.einsum("ijl,ijl->", q, q))
+    return s / (p.nx * p.ny * p.nz)
 
 
 def ensemble_member(i, sizes=((33, 33, 97), (49, 49, 145), (65, 65, 193), (97, 97, 289))):

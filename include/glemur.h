@@ -120,12 +120,15 @@ typedef struct gm_step_info {
 #define GM_FIELD_DETJAC  3   /*  P  det of the grid Jacobian               */
 #define GM_FIELD_J       4   /* 3N  current density, inner AoS             */
 #define GM_FIELD_GRADP   5   /* 3N  pressure gradient, inner AoS           */
-#define GM_FIELD_KK      6   /* 18N six stage slopes, inner AoS, back to back */
+#define GM_FIELD_KK      6   /* 18N six stage slopes, inner AoS, back to back;
+                                 36N with inertia: [stage][point][x0 x1 x2 u0 u1 u2]
+                                 (src/rungeKutta.cu:236-244)                    */
 #define GM_FIELD_XB_NEW  7   /* 3P  candidate positions of the last attempt */
 #define GM_FIELD_XB_LO   8   /* 3P  compensation word of the positions (F32K; zeros in F64) */
 #define GM_FIELD_CELLVOL 9   /*  N  signed cell volume, forward differences (src/maths.cu:443-494) */
 #define GM_FIELD_CONVEXITY 10 /* N  +1 / -1: all 8 octant wedges positive (src/maths.cu:573-640) */
 #define GM_FIELD_WEDGEMIN 11 /*  N  smallest oriented octant wedge (src/maths.cu:497-570) */
+#define GM_FIELD_UU      12  /* 3N  velocity, inner AoS (inertia only; src/global.h:143) */
 
 typedef struct gm_ctx gm_ctx;
 
@@ -164,6 +167,10 @@ int gm_create(const gm_params *p, int precision, gm_ctx **ctx);
 int gm_destroy(gm_ctx *ctx);
 /* H2D of B0 and xb plus the device copies of src/main.cu:181-215. */
 int gm_upload(gm_ctx *ctx, const double *B0, const double *xb, double t, double dt);
+/* H2D of the velocity of inner points, inner AoS 3N (cudaMemcpy of h.uu,
+ * src/main.cu:216-222).  Contexts with inertia only; gm_upload resets the
+ * velocity to zero. */
+int gm_upload_velocity(gm_ctx *ctx, const double *uu);
 /* D2H of one field into reference layout (src/diagnostics.cu:145-152). */
 int gm_download(gm_ctx *ctx, int field, double *host);
 int gm_get_time(gm_ctx *ctx, double *t, double *dt);
@@ -328,13 +335,15 @@ int gm_autotune(gm_ctx *ctx, int reps, double ms_best[6]);
 /* writeB0 (src/io.cu:167-205): data/B0.vtk in dir. */
 int gm_write_b0(const char *dir, const gm_params *p, const double *B0, int precision);
 /* dumpState (src/io.cu:209-386): n = -1 writes save.vtk, else dump<n>.vtk.
- * Arrays in reference layout; J is inner AoS; NULL skips optional fields. */
+ * Arrays in reference layout; uu and J are inner AoS (uu written as
+ * "ufield" when p->inertia); NULL skips optional fields. */
 int gm_dump_state(const char *dir, const gm_params *p, double t, double dt, int n,
-                  const double *xb, const double *B, const double *J,
+                  const double *xb, const double *B, const double *uu, const double *J,
                   const double *detJac, const double *cellVol, const double *convexity,
                   const double *wedgeMin, int precision);
-/* readState (src/io.cu:390-499): xb, t, dt from save.vtk, B0 from B0.vtk. */
-int gm_read_state(const char *dir, const gm_params *p, double *xb, double *B0,
+/* readState (src/io.cu:390-499): xb, t, dt (and uu, inner AoS, when
+ * p->inertia) from save.vtk, B0 from B0.vtk.  uu may be NULL otherwise. */
+int gm_read_state(const char *dir, const gm_params *p, double *xb, double *B0, double *uu,
                   double *t, double *dt, int precision);
 /* writeTs (src/io.cu:542-678); header when t == 0 (echoed to stdout when
  * echo != 0).  red may be NULL for the header.  jxb_jb_col != 0 appends the

@@ -256,3 +256,43 @@ def test_full_size_fused_equals_per_kernel_fp64(g, medium):
         if s.accepted:
             b.accept()
             assert np.array_equal(b.download(g.FIELD_XB), x_f)
+
+
+def test_large_pressure_braid_f32k_tracks_fp64(g):
+    """The 257 x 257 x 769 braid with seeded amplitudes and pressure on
+    (51 M inner points, the F32K positions near their 32-bit offset limit):
+    20 steps from the initial state in FP64 and in F32K at half the dt the
+    controller reaches in 12 FP64 attempts, so every attempt is accepted in
+    both and the runs sit at the same times.  The energy differs by at most
+    1e-6 relative, the relaxation criterion by 1e-3, the positions by 1e-6
+    of the total displacement, and the F32K energy never rises."""
+    from glemur_amd import workloads
+    p = workloads.large()
+    B0, xb = g.init_state(p)
+    with g.Context(p, g.PREC_F64) as a:
+        a.upload(B0, xb, 0.0, p.dt0)
+        a.run(max_attempts=12, batch=12)
+        dt = 0.5 * a.time()[1]
+    n = 20
+    print(f"fixed dt {dt:.4e}")
+    with g.Context(p, g.PREC_F64) as a, g.Context(p, g.PREC_F32K) as b:
+        a.upload(B0, xb, 0.0, dt)
+        b.upload(B0, xb, 0.0, dt)
+        for it in range(n):
+            a.set_time(a.time()[0], dt)
+            b.set_time(b.time()[0], dt)
+            sa, sb = a.step(), b.step()
+            assert sa.accepted and sb.accepted and sa.maxDelta < 0.1 * p.maxError, it
+        la, lb = a.step_log(n), b.step_log(n)
+        xa, xf = a.download(g.FIELD_XB), b.download(g.FIELD_XB)
+    ea = np.array([r.energy for r in la])
+    eb = np.array([r.energy for r in lb])
+    ra = np.array([r.ratio for r in la])
+    rb = np.array([r.ratio for r in lb])
+    de = np.max(np.abs(eb - ea) / ea)
+    dr = np.max(np.abs(rb - ra) / ra)
+    dx = np.max(np.abs(xf - xa)) / np.max(np.abs(xa - xb))
+    print(f"energy {ea[0]:.8e} -> {ea[-1]:.8e}; max rel. difference energy {de:.2e} criterion {dr:.2e}; "
+          f"positions {dx:.2e} of the displacement")
+    assert de < 1e-6 and dr < 1e-3 and dx < 1e-6
+    assert np.all(np.diff(eb) <= 0) and eb[-1] < eb[0]

@@ -276,3 +276,97 @@ def test_jmethod_variants_fields_and_steps_bitwise(g, method):
         assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), f"positions ({method})")
     with pytest.raises(g.GlemurError):
         g.Context(p, g.PREC_F32K)
+
+
+def test_braid_33x33x97_200_steps_tracks_oracle(g):
+    """The plain FP64 workload on its full size: the 33 x 33 x 97 braid of
+    line-tied faces, pressure off, friction 1, adaptive Cash-Karp at
+    tolerance 1e-5, 200 accepted steps (src/main.cu:259-384).  gm_run drives
+    the device controller without host round trips; the oracle runs the
+    same loop on its own dt sequence.  dt agrees to 2 ulp per step, and the
+    runs separate only by rounding that the controller feeds back: the error
+    estimate is a difference of slopes cancelling by about four orders of
+    magnitude, and dt takes 0.2 of its relative error.  Bounds: every
+    attempt's decision equal; dt and the relaxation criterion within 1e-10
+    relative; energy within 1e-12; error estimate within 1e-7 of the
+    tolerance; final positions within 1e-10 of the box size."""
+    from glemur_amd import workloads
+    p = workloads.small()
+    assert (p.nx, p.ny, p.nz, p.nCycle, p.maxError) == (33, 33, 97, 200, 1e-5)
+    B0, xb = g.init_state(p)
+    sim = oracle_sim(p, B0, xb)
+    rows = []
+    while sum(r[0] for r in rows) < 200:
+        sim.fields()
+        e, crit = sim.b2(), sim.jxbmax_jbmax()
+        acc, dtu, md, nan = sim.step()
+        assert not nan
+        rows.append((acc, dtu, md, e, crit))
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, p.dt0)
+        st = ctx.run(max_attempts=10 ** 6, n_cycle=200, batch=25)
+        log = ctx.step_log(len(rows) + 8)
+        xg = ctx.download(g.FIELD_XB)
+    assert len(log) == len(rows), (len(log), len(rows))
+    o = np.array([r[1:] for r in rows])
+    d = np.array([(r.dt_used, r.maxDelta, r.energy, r.ratio) for r in log])
+    assert [bool(r.accepted) for r in log] == [r[0] for r in rows]
+    rel = np.abs(d - o) / np.abs(o)
+    ddt, dmd, den, dcr = rel[:, 0].max(), np.abs(d[:, 1] - o[:, 1]).max(), rel[:, 2].max(), rel[:, 3].max()
+    dx = np.max(np.abs(xg - sim.arr("xb")))
+    n_rej = sum(not r[0] for r in rows)
+    print(f"{len(rows)} attempts ({n_rej} rejected), t = {st.t:.6f}, energy {rows[0][3]:.10e} -> "
+          f"{log[-1].energy:.10e}, criterion {rows[0][4]:.4e} -> {log[-1].ratio:.4e}; max differences: "
+          f"dt {ddt:.2e} rel, error estimate {dmd:.2e} abs, energy {den:.2e} rel, criterion {dcr:.2e} rel, "
+          f"positions {dx:.2e} abs")
+    assert ddt <= 1e-10 and dcr <= 1e-10 and den <= 1e-12
+    assert dmd <= 1e-7 * p.maxError
+    assert dx <= 1e-10 * p.Lz
+    assert log[-1].energy < log[0].energy
+
+
+@pytest.mark.parametrize("case", [{}, {"pressure": True, "beta": 0.05}, {"zPeri": True}])
+def test_inertia_steps_bitwise(g, case):
+    """inertia = t (src/rungeKutta.cu:112-353, inertia branches) on the FP64
+    per-kernel path, from a seeded nonzero velocity: 25 attempted steps
+    (gm_step) with every decision, error estimate and dt bit for bit; the
+    36N stage slopes, positions and velocity after each of the first three
+    attempts and at the end equal the oracle's; U2 and UMax of the result
+    match under the reference's redU2/redUMax flags (U2 summed in another
+    order: 1e-13); gm_run continues the same trajectory; F32K contexts
+    refuse inertia."""
+    p = small_braid(inertia=True, nu=0.6, redU2=True, redUMax=True, **case)
+    B0, xb = evolved_state(p, 0)
+    rng = np.random.default_rng(12)
+    u0 = 0.05 * rng.standard_normal(3 * p.ninner)
+    sim = oracle_sim(p, B0, xb)
+    sim.arr("uu")[:] = u0
+    n_rej = 0
+    with g.Context(p, g.PREC_F64) as ctx:
+        ctx.upload(B0, xb, 0.0, p.dt0)
+        assert np.all(ctx.download(g.FIELD_UU) == 0)      # upload starts at rest
+        ctx.upload_velocity(u0)
+        assert_bitwise(ctx.download(g.FIELD_UU), u0, "uploaded velocity")
+        sim.set_dt(p.dt0)
+        for it in range(25):
+            s = ctx.step()
+            acc, dtu, md, _ = sim.step()
+            assert s.dt_used == dtu and s.maxDelta == md and bool(s.accepted) == acc and s.dt == sim.dt, it
+            n_rej += not acc
+            if it < 3 or it == 24:
+                assert_bitwise(ctx.download(g.FIELD_KK), sim.arr("kk"), f"stage slopes at attempt {it}")
+                assert_bitwise(ctx.download(g.FIELD_XB), sim.arr("xb"), f"positions at attempt {it}")
+                assert_bitwise(ctx.download(g.FIELD_UU), sim.arr("uu"), f"velocity at attempt {it}")
+        ctx.fields()
+        r = ctx.reductions()
+        sim.fields()
+        assert r.UMax == sim.umax() and r.U2 == pytest.approx(sim.u2(), rel=1e-13)
+        assert r.U2 > 0 and r.B2 == pytest.approx(sim.b2(), rel=1e-13)
+        st = ctx.run(max_attempts=6, batch=3)
+        for _ in range(6):
+            sim.step()
+        assert st.n_attempt == 31
+        assert_bitwise(ctx.download(g.FIELD_UU), sim.arr("uu"), "velocity after gm_run")
+    assert n_rej >= 1, "the trajectory should exercise the reject path"
+    with pytest.raises(g.GlemurError):
+        g.Context(p, g.PREC_F32K)

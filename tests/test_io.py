@@ -132,6 +132,45 @@ def test_save_and_restart_round_trip(tmp_path):
     assert t == 0.75 and dt == 1.5e-4
 
 
+def test_velocity_dump_and_restart(tmp_path):
+    """inertia = t: the velocity is written as "VECTORS ufield" between
+    bfield and jfield with zero ghosts (src/io.cu:269-290) and read back on
+    restart (src/io.cu:466-479) bitwise; no ufield without inertia."""
+    p = small_params(inertia=True, dumpJ=True)
+    os.makedirs(tmp_path / "data")
+    rng = np.random.default_rng(4)
+    B0, xb = g.init_state(p)
+    B = rng.standard_normal(3 * p.npad)
+    uu = rng.standard_normal(3 * p.ninner)
+    J = rng.standard_normal(3 * p.ninner)
+    g.write_b0(tmp_path, p, B0, 64)
+    g.dump_state(tmp_path, p, 0.5, 2e-3, -1, xb, B, J, uu=uu)
+    r = Reader(tmp_path / "data" / "save.vtk")
+    for _ in range(5):
+        r.line()
+    expect_param_block(r, p, 0.5, 2e-3, ">f8", "double")
+    assert r.line() == "POINTS %9i double" % p.npad
+    r.block(">f8", 3 * p.npad)
+    assert r.line() == "POINT_DATA %9i" % p.npad
+    assert r.line() == "VECTORS bfield double"
+    np.testing.assert_array_equal(r.block(">f8", 3 * p.npad), B)
+    assert r.line() == "VECTORS ufield double"
+    uf = r.block(">f8", 3 * p.npad).reshape(p.nz + 2, p.ny + 2, p.nx + 2, 3)
+    want = np.zeros_like(uf)
+    want[1:-1, 1:-1, 1:-1] = uu.reshape(p.nz, p.ny, p.nx, 3)
+    np.testing.assert_array_equal(uf, want)
+    assert r.line() == "VECTORS jfield double"
+    xr, Br, t, dt, ur = g.read_state(tmp_path, p, 64)
+    assert np.array_equal(xr, xb) and np.array_equal(Br, B0) and np.array_equal(ur, uu)
+    assert t == 0.5 and dt == 2e-3
+    # a save without the velocity cannot restart an inertia run
+    q = small_params(dumpJ=True)
+    g.dump_state(tmp_path, q, 0.5, 2e-3, -1, xb, B, J, uu=uu)
+    assert b"ufield" not in open(tmp_path / "data" / "save.vtk", "rb").read()
+    with pytest.raises(g.GlemurError):
+        g.read_state(tmp_path, p, 64)
+
+
 def test_read_state_missing_file_is_an_error(tmp_path):
     p = small_params()
     with pytest.raises(g.GlemurError):
