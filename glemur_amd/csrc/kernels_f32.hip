@@ -487,22 +487,26 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 A.edet[a] = detv;
             }
             if (DIAG && A.diag) {
+                // squared magnitudes in FP64 (products of floats, exact but
+                // for the sums); the maxima are kept squared and rooted once
+                // per thread after the march; the two per-point quotients
+                // take the hardware reciprocal in FP32
                 double Bd[3] = {(double)Bc[0], (double)Bc[1], (double)Bc[2]};
                 double Jd[3] = {(double)Jv[0], (double)Jv[1], (double)Jv[2]}, jxb[3];
                 cross3(Jd, Bd, jxb);
-                double b2 = Bd[0] * Bd[0] + Bd[1] * Bd[1] + Bd[2] * Bd[2];
-                double nc = sqrt(jxb[0] * jxb[0] + jxb[1] * jxb[1] + jxb[2] * jxb[2]);
-                double nj = sqrt(Jd[0] * Jd[0] + Jd[1] * Jd[1] + Jd[2] * Jd[2]);
-                double v = b2 * detv;
-                int bound = (ii == 0) + (jj == 0) + (kk == 0) + (ii == g.nx - 1) + (jj == g.ny - 1) + (kk == g.nz - 1);
-                v /= (1 << bound);
-                e_sum += v;
-                jmax = nb_max(jmax, nj);
-                r1 = nb_max(r1, nc / b2);
-                double den = nj * sqrt(b2);
-                r2 = nb_max(r2, den > 0 ? nc / den : 0.0);
-                r3 = nb_max(r3, nc);
-                r4 = nb_max(r4, den);
+                const double b2 = Bd[0] * Bd[0] + Bd[1] * Bd[1] + Bd[2] * Bd[2];
+                const double nc2 = jxb[0] * jxb[0] + jxb[1] * jxb[1] + jxb[2] * jxb[2];
+                const double nj2 = Jd[0] * Jd[0] + Jd[1] * Jd[1] + Jd[2] * Jd[2];
+                const double den2 = nj2 * b2;
+                const int bound = (ii == 0) + (jj == 0) + (kk == 0) + (ii == g.nx - 1) + (jj == g.ny - 1) +
+                                  (kk == g.nz - 1);
+                e_sum += ldexp(b2 * detv, -bound);   // |B|^2 det / 2^bound, exact scaling
+                jmax = nb_max(jmax, nj2);
+                const float fnc2 = (float)nc2, fb2 = (float)b2;
+                r1 = nb_max(r1, (double)(fnc2 * __builtin_amdgcn_rcpf(fb2 * fb2)));
+                r2 = nb_max(r2, den2 > 0 ? (double)(fnc2 * __builtin_amdgcn_rcpf((float)den2)) : 0.0);
+                r3 = nb_max(r3, nc2);
+                r4 = nb_max(r4, den2);
             }
             if constexpr (!FINAL) {
                 if constexpr (stage >= 0) {
@@ -559,6 +563,13 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         }
     }
 
+    if (DIAG && A.diag) {   // the maxima were kept squared
+        jmax = sqrt(jmax);
+        r1 = sqrt(r1);
+        r2 = sqrt(r2);
+        r3 = sqrt(r3);
+        r4 = sqrt(r4);
+    }
     // ---- block partials (fixed order: deterministic)
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (FINAL) {
