@@ -279,7 +279,8 @@ def main():
                    "tile_tx_ty_cz_workgroups": list(f32["tile"]), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_f32,
-                     "kernel": "k_stage<true,512,n,false> (fused Cash-Karp stage, F32K), mean of stages 0-5",
+                     "kernel": "k_stage32<NT,n,false> (fused Cash-Karp stage n, F32K; NT = 1024 for stages 0-4, "
+                               "768 for stage 5 at the default tiles), mean of stages 0-5",
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "algorithmic_bytes_per_point": bpp,
                      "mean_launch_ms": f32["ms_stage"],

@@ -75,7 +75,7 @@ void launch_fused32(hipStream_t s, const FusedArgs &A);
 // workgroups per CU its stage kernel is built for (stage -1: fields only).
 int threads32(int rect);
 int blocks_per_cu32(int rect, int stage, int press);
-// Largest rect the FP32 kernel of this stage is built for (512 or 1024).
+// Largest rect the FP32 kernel of this stage is built for (512, 768 or 1024).
 int max_rect32(int stage, int press);
 void launch_controller(hipStream_t s, const CtlArgs &A);
 // Ensemble launches: every workgroup of n instances in one launch (FP32

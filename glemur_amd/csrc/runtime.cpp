@@ -1189,14 +1189,14 @@ extern "C" int gm_autotune(gm_ctx *c, int reps, double ms_best[6])
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    const int bounds[5] = {0, 256, 384, 512, 1024};
+    const int bounds[6] = {0, 256, 384, 512, 768, 1024};
     int rc = GM_OK;
     for (int k = 0; k < 7 && rc == GM_OK; ++k) {
         const int stage = k == 6 ? -1 : k;
         const int cap = stage_max_rect(c, k);
         int bTX = c->TX[k], bTY = c->TY[k], bCZ = c->CZ[k];
         double best = 1e30;
-        for (int w = 0; w < 4 && rc == GM_OK; ++w) {
+        for (int w = 0; w < 5 && rc == GM_OK; ++w) {
             if (bounds[w] >= cap) break;
             int tx, ty;
             if (!best_rect(c, bounds[w], std::min(bounds[w + 1], cap), tx, ty)) continue;
