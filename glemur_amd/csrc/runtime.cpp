@@ -119,10 +119,27 @@ void set_tile_k(gm_ctx *c, int k, int tx, int ty, int cz)
     c->nblocks[k] = c->tiles_x[k] * c->tiles_y[k] * c->nchunks[k];
 }
 
+// z chunks per tile column: the workgroups run in rounds of `slots`
+// (resident workgroups over all CUs) and each marches its chunk plus five
+// planes of pipeline fill, so the launch takes about
+// ceil(chunks x cols / slots) x (ceil(nz / chunks) + 5) plane steps; the
+// fewest such steps, chunks never thinner than 8 planes
+int best_chunks(int nz, int cols, int slots)
+{
+    int best = 1;
+    long long best_cost = -1;
+    for (int nch = 1; nch <= std::max(1, nz / 8); ++nch) {
+        const long long rounds = ((long long)nch * cols + slots - 1) / slots;
+        const long long cost = rounds * ((nz + nch - 1) / nch + 5);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = nch; }
+    }
+    return best;
+}
+
 // Tile choice per stage kernel: among the rects (TX+4)(TY+4) that fit the
-// kernel's workgroup, the fewest loaded points per output point; then z
-// chunks so that one round over the CUs (`target` x workgroups per CU) is
-// filled, never thinner than 8 planes.
+// kernel's workgroup, the fewest loaded points per output point; then the z
+// chunks of best_chunks with `target` x the kernel's workgroups per CU
+// resident at a time.
 void auto_tiles(gm_ctx *c, int target)
 {
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
@@ -139,9 +156,7 @@ void auto_tiles(gm_ctx *c, int target)
                 if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
             }
         set_tile_k(c, k, bx, by, nz);
-        const int cols = c->tiles_x[k] * c->tiles_y[k];
-        int nch = std::max(1, target * stage_blocks_per_cu(c, k) / cols);
-        nch = std::min(nch, std::max(1, nz / 8));
+        const int nch = best_chunks(nz, c->tiles_x[k] * c->tiles_y[k], target * stage_blocks_per_cu(c, k));
         set_tile_k(c, k, bx, by, (nz + nch - 1) / nch);
     }
 }
@@ -1200,11 +1215,16 @@ extern "C" int gm_autotune(gm_ctx *c, int reps, double ms_best[6])
             if (bounds[w] >= cap) break;
             int tx, ty;
             if (!best_rect(c, bounds[w], std::min(bounds[w + 1], cap), tx, ty)) continue;
-            for (int rounds = 1; rounds <= 2; ++rounds) {
-                set_tile_k(c, k, tx, ty, c->p.nz);
-                const int cols = c->tiles_x[k] * c->tiles_y[k];
-                int nch = std::max(1, c->ncu * rounds * stage_blocks_per_cu(c, k) / cols);
-                nch = std::min(nch, std::max(1, c->p.nz / 8));
+            // z chunks: the plane-step model's choice, and one or two full
+            // rounds over the CUs
+            set_tile_k(c, k, tx, ty, c->p.nz);
+            const int cols = c->tiles_x[k] * c->tiles_y[k], slots = c->ncu * stage_blocks_per_cu(c, k);
+            int cand[3] = {best_chunks(c->p.nz, cols, slots), slots / cols, 2 * slots / cols};
+            for (int q = 0; q < 3; ++q) {
+                const int nch = cand[q] = std::max(1, std::min(cand[q], std::max(1, c->p.nz / 8)));
+                bool timed = false;
+                for (int r = 0; r < q; ++r) timed |= cand[r] == nch;
+                if (timed) continue;
                 set_tile_k(c, k, tx, ty, (c->p.nz + nch - 1) / nch);
                 if ((rc = ensure_partials(c))) break;
                 gmd::FusedArgs A = fused_args(c, stage, 0, stage == 0 ? 1 : 0, 0);
