@@ -684,11 +684,13 @@ void launch_1024(hipStream_t s, const FusedArgs &A)
 
 int threads32(int rect) { return rect <= 256 ? 256 : rect <= 384 ? 384 : rect <= 512 ? 512 : rect <= 768 ? 768 : 1024; }
 
-// 768-thread workgroups (12 waves, 168 VGPRs) for the kernels without
-// pressure: the final stage then runs three waves per SIMD instead of two
-// (266 instead of 272 us at 129 x 129 x 385); the pressure kernels measured
-// 22% slower at 768 than at 512 threads, so they stay at 512
-constexpr bool fits768(int, bool press) { return !press; }
+// 768-thread workgroups (12 waves, 168 VGPRs, three waves per SIMD) for
+// the kernels that fit that budget without spilling: all but the pressure
+// variants of the final stage and of the fields pass.  The final stage
+// without pressure: 266 instead of 272 us at 129 x 129 x 385; the pressure
+// stages 0-4 at 257 x 257 x 769: 4-11% faster than in 512-thread
+// workgroups
+constexpr bool fits768(int stage, bool press) { return !(press && (stage == 5 || stage == 6)); }
 
 int max_rect32(int stage, int press)
 {
@@ -746,7 +748,7 @@ void launch_fused32(hipStream_t s, const FusedArgs &A)
     case 256: launch_nt32<256>(s, A); break;
     case 384: launch_nt32<384>(s, A); break;
     case 512: launch_nt32<512>(s, A); break;
-    case 768: launch_stage32<768, false>(s, A); break;   // offered only without pressure
+    case 768: launch_nt32<768>(s, A); break;
     default: launch_1024(s, A); break;   // the runtime offers 1024 only where fits1024
     }
 }
