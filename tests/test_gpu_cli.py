@@ -28,24 +28,28 @@ PARAMS = """&comp
 /
 &run
     nCycle = {ncycle}, maxError = 1e-5, dt0 = 3e-4, dtMin = 1e-9
-    epsilonProf = f, pressure = f, zUpdate = t, jMethod = "Stokes"
+    epsilonProf = f, pressure = f, zUpdate = t, jMethod = "Stokes"{run_extra}
 /
 &io
     nTs = 2, dtDump = 0.05, dtSave = 0.1
     dumpJ = t, dumpDetJac = t, dumpCellVol = t, dumpConvexity = t, dumpWedgeMin = t
     redB2 = t, redJMax = t, redJxB_B2Max = t, redConvex = t, redWedgeMin = t, redEpsilonStar = t
-    redErrB_1ez = t
+    redErrB_1ez = t{io_extra}
 /
 """
 
 
-def run_cli(d, *args, restart=False, ncycle=30):
-    (d / "params.in").write_text(PARAMS.format(restart="t" if restart else "f", ncycle=ncycle))
+def run_cli(d, *args, restart=False, ncycle=30, run_extra="", io_extra="", ok=True):
+    (d / "params.in").write_text(PARAMS.format(restart="t" if restart else "f", ncycle=ncycle, run_extra=run_extra,
+                                               io_extra=io_extra))
     os.makedirs(d / "data", exist_ok=True)
     print("running glemur", args, "in", d, flush=True)
     r = subprocess.run([EXE, *args], cwd=d, capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stdout + r.stderr
-    return r.stdout
+    if ok:
+        assert r.returncode == 0, r.stdout + r.stderr
+    else:
+        assert r.returncode != 0, r.stdout
+    return r.stdout + r.stderr
 
 
 def ts_rows(d):
@@ -96,3 +100,39 @@ def test_restart_continues(require_gpu, tmp_path):
     assert min(t_new) > t_save
     B0_init, _ = g.init_state(p)
     assert np.array_equal(B0, B0_init)
+
+
+INERTIA = dict(run_extra=", inertia = t, nu = 0.5", io_extra=", redU2 = t, redUMax = t")
+
+
+def test_inertia_run_and_restart(require_gpu, tmp_path):
+    """inertia = t through the executable: the velocity starts at rest and
+    is driven by J x B (U2 and UMax grow from zero) and save.vtk holds a
+    ufield block.  The save keeps the whole FP64 state (positions with their
+    ghosts, velocity, t, dt), so 30 steps plus a restart for 10 more print
+    the same time-series rows as 40 steps in one run.  FP32 runs refuse
+    inertia."""
+    one, two = tmp_path / "one", tmp_path / "two"
+    for d in (one, two):
+        os.makedirs(d)
+    run_cli(one, "-p", "64", ncycle=40, **INERTIA)
+    run_cli(two, "-p", "64", ncycle=30, **INERTIA)
+    head, rows = ts_rows(two)
+    col = lambda name, rs: [float(r[head.index(name) - (1 if head[0] == "#" else 0)]) for r in rs]
+    u2, umax = col("U2", rows), col("UMax", rows)
+    assert u2[-1] > 0 and umax[-1] > 0 and max(u2) > u2[0]
+    assert b"VECTORS ufield" in (two / "data" / "save.vtk").read_bytes()
+    import glemur_amd as g
+    p = g.params_read(two / "params.in")
+    xb, B0, t_save, dt_save, uu = g.read_state(two, p, 64)
+    assert np.abs(uu).max() > 0
+    run_cli(two, "-p", "64", restart=True, ncycle=10, **INERTIA)
+    _, rows_two = ts_rows(two)
+    _, rows_one = ts_rows(one)
+    cont = rows_two[len(rows):]
+    want = [r for r in rows_one if int(r[0]) >= 30]
+    assert len(cont) == len(want) == 5
+    # the time index restarts at 0; every other column is printed identically
+    assert [r[1:] for r in cont] == [r[1:] for r in want], (cont, want)
+    out = run_cli(tmp_path, "-p", "32", ncycle=5, ok=False, **INERTIA)
+    assert "inertia" in out
