@@ -296,3 +296,40 @@ def test_large_pressure_braid_f32k_tracks_fp64(g):
           f"positions {dx:.2e} of the displacement")
     assert de < 1e-6 and dr < 1e-3 and dx < 1e-6
     assert np.all(np.diff(eb) <= 0) and eb[-1] < eb[0]
+
+
+def test_medium_braid_f32k_tracks_fp64_over_3000_steps(g, medium):
+    """The 129 x 129 x 385 braid relaxed for 3000 steps at a fixed dt of
+    2e-3 (t = 6, every attempt accepted in both precisions) in FP64 and in
+    F32K.  At the end: energy within 1e-5 relative (the accuracy target of
+    the compensated path), the relaxation criterion within 1e-3, B within
+    1e-5 and J within 1e-3 of their maxima, positions within 1e-5 of the
+    total displacement; the F32K energy never rises."""
+    p, B0, xb = medium
+    n, dt = 3000, 2e-3
+    out = {}
+    for prec in ("F64", "F32K"):
+        with g.Context(p, getattr(g, "PREC_" + prec)) as c:
+            c.upload(B0, xb, 0.0, dt)
+            energies = []
+            for it in range(n):
+                c.set_time(c.time()[0], dt)
+                s = c.step()
+                assert s.accepted and s.maxDelta < p.maxError, (prec, it, s.maxDelta)
+                if it % 100 == 0:
+                    energies.append(c.step_log(1)[0].energy)
+            c.fields()
+            r = c.reductions()
+            out[prec] = dict(x=c.download(g.FIELD_XB), B=c.download(g.FIELD_B), J=c.download(g.FIELD_J),
+                             energy=r.B2, crit=r.JxBMax_JBMax, e=np.array(energies), t=c.time()[0])
+    a, b = out["F64"], out["F32K"]
+    m = inner_mask(p)
+    de = abs(b["energy"] - a["energy"]) / a["energy"]
+    dc = abs(b["crit"] - a["crit"]) / a["crit"]
+    dB = relerr(b["B"].reshape(-1, 3)[m], a["B"].reshape(-1, 3)[m])
+    dJ = relerr(b["J"], a["J"])
+    dx = np.max(np.abs(b["x"] - a["x"])) / np.max(np.abs(a["x"] - xb))
+    print(f"t = {a['t']:.3f}: energy {a['e'][0]:.8e} -> {a['energy']:.8e} (F64), criterion {a['crit']:.4e}; "
+          f"F32K vs FP64: energy {de:.2e}, criterion {dc:.2e}, B {dB:.2e}, J {dJ:.2e}, positions {dx:.2e}")
+    assert de < 1e-5 and dc < 1e-3 and dB < 1e-5 and dJ < 1e-3 and dx < 1e-5
+    assert np.all(np.diff(b["e"]) <= 0)
