@@ -35,7 +35,7 @@ RK-stages per second over the attempts.
 time_to_relax: wall time from the initial state of the 129 x 129 x 385
 braid to max|JxB| / max(|J||B|) <= 0.03, cold and with the learned warm
 start (U-Net inference, up-sampling, checks and 50 settle attempts
-included).
+included), and the energies of the two states reached.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -160,6 +160,8 @@ def time_to_relax(g, theta):
     return {"theta": theta, "cold_s": cold["seconds"], "cold_attempts": cold["attempts"],
             "warm_s": warm["seconds"], "warm_attempts": warm["attempts"], "warm_used": warm["warm_used"],
             "warm_start_ms": warm["warm_ms"], "gain": cold["seconds"] / warm["seconds"] if warm["converged"] else None,
+            "energy_cold": cold["energy"], "energy_warm": warm["energy"],
+            "energy_rel_diff": (warm["energy"] - cold["energy"]) / cold["energy"],
             "weights": "glemur_amd/data/unet_braid65x65x193.npz (2048 seeded 65x65x193 braids, 2000 attempts each)"}
 
 

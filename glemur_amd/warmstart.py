@@ -87,7 +87,7 @@ def time_to_relax(p, theta, max_attempts, unet=None, settle=50, dt0=0.02, batch=
         c.sync()
         secs = time.perf_counter() - t0
     out = {"seconds": secs, "attempts": int(st.n_attempt), "t": st.t, "converged": bool(st.converged),
-           "ratio": st.ratio}
+           "ratio": st.ratio, "energy": st.energy}
     if info is not None:
         out.update({"warm_used": bool(info.used), "ratio_cold": info.ratio_cold, "ratio_applied": info.ratio_applied,
                     "ratio_after_settle": info.ratio_warm, "energy_cold": info.energy_cold,
@@ -132,7 +132,8 @@ def main(argv=None):
         cold = time_to_relax(p, a.theta, 20000)
         warm = time_to_relax(p, a.theta, 20000, unet=u, settle=a.settle)
         r = {"case": name, "theta": a.theta, "cold": cold, "warm": warm,
-             "gain": cold["seconds"] / warm["seconds"] if warm["converged"] else None}
+             "gain": cold["seconds"] / warm["seconds"] if warm["converged"] else None,
+             "energy_rel_diff": (warm["energy"] - cold["energy"]) / cold["energy"]}
         print(json.dumps(r), flush=True)
         res.append(r)
     u.close()
