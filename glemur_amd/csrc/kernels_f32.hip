@@ -279,7 +279,6 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 // central differences of x along each axis, one neighbour
                 // pair at a time; they give the Jacobian here and, kept for
                 // one plane, the current at this point in the next iteration
-                float jac[3][3];
                 const float hx = 0.5f * (float)g.dx1, hy = 0.5f * (float)g.dy1, hz = 0.5f * (float)g.dz1;
                 {
                     Pt3 a, b;
@@ -298,13 +297,17 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) {
-                    drq[1][2][l] = dif(xzp, xzm, l);
-                    jac[l][0] = drq[1][0][l] * hx;
-                    jac[l][1] = drq[1][1][l] * hy;
-                    jac[l][2] = drq[1][2][l] * hz;
-                }
-                detv = cauchy32(jac, b0q[0], -1, Bv);
+                for (int l = 0; l < 3; ++l) drq[1][2][l] = dif(xzp, xzm, l);
+                // jac = dr diag(hx, hy, hz): det(jac) = det(dr) hx hy hz
+                // (det(dr) from its rows, the transpose of jac's columns),
+                // and jac B0 = dr (h * B0), so the spacing scales the three
+                // components of B0 instead of the nine of dr
+                detv = det32(drq[1]) * ((double)hx * (double)hy * (double)hz);
+                const float rd = __builtin_amdgcn_rcpf((float)detv);
+                const float bs[3] = {b0q[0][0] * hx, b0q[0][1] * hy, b0q[0][2] * hz};
+#pragma unroll
+                for (int l = 0; l < 3; ++l)
+                    Bv[l] = (drq[1][0][l] * bs[0] + drq[1][1][l] * bs[1] + drq[1][2][l] * bs[2]) * rd;
             } else if (!zg && !A.face_table) {
                 // x or y face, line-tied: tangential columns from the ghost
                 // layer, normal column from the inner neighbour and the
@@ -417,9 +420,10 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 const int a1 = (a0 + 1) % 3, a2 = (a0 + 2) % 3;
                 float C[3];
                 cross3(dr[a1], dr[a2], C);
-                Jv[0] += C[1] * db[a0][2] - C[2] * db[a0][1];
-                Jv[1] += C[2] * db[a0][0] - C[0] * db[a0][2];
-                Jv[2] += C[0] * db[a0][1] - C[1] * db[a0][0];
+                // Jv += C x db[a0], two fused products per component
+                Jv[0] = fmaf(C[1], db[a0][2], fmaf(-C[2], db[a0][1], Jv[0]));
+                Jv[1] = fmaf(C[2], db[a0][0], fmaf(-C[0], db[a0][2], Jv[1]));
+                Jv[2] = fmaf(C[0], db[a0][1], fmaf(-C[1], db[a0][0], Jv[2]));
             }
             {
                 // det[dx dy dz] = 8 hx hy hz det(jac), det(jac) from FP64
