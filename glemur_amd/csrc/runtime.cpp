@@ -1338,6 +1338,31 @@ void batch_enqueue_step(gm_batch *b)
 
 }  // namespace
 
+// Ensemble z chunks: one launch runs every member's workgroups, so it lasts
+// as long as the longest march.  Per stage kernel the members share one
+// chunk length L (a member shorter than L takes one chunk): the fewest
+// ceil(workgroups / resident slots) x (L + 5) plane steps.
+static void batch_chunks(gm_batch *b)
+{
+    gm_ctx *c0 = b->inst[0];
+    for (int k = 0; k < 7; ++k) {
+        // the batch stage kernel runs 512-thread workgroups
+        const long long slots = (long long)c0->ncu * gmd::blocks_per_cu32(512, k == 6 ? -1 : k, b->press);
+        int maxnz = 1;
+        for (gm_ctx *c : b->inst) maxnz = std::max(maxnz, c->p.nz);
+        int bestL = maxnz;
+        long long best = -1;
+        for (int L = std::min(8, maxnz); L <= maxnz; ++L) {
+            long long wg = 0;
+            for (gm_ctx *c : b->inst)
+                wg += (long long)c->tiles_x[k] * c->tiles_y[k] * ((c->p.nz + L - 1) / L);
+            const long long cost = ((wg + slots - 1) / slots) * (L + 5);
+            if (best < 0 || cost < best) { best = cost; bestL = L; }
+        }
+        for (gm_ctx *c : b->inst) set_tile_k(c, k, c->TX[k], c->TY[k], bestL);
+    }
+}
+
 extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm_batch **out)
 {
     if (!params || !out || n < 1) return fail(GM_ERR_ARG, "gm_batch_create: bad arguments");
@@ -1358,12 +1383,13 @@ extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm
         if ((r = gm_create(&params[i], precision, &c))) return bail(r);
         b->inst.push_back(c);
         c->rect_cap = 512;
-        // one round over the CUs for the whole ensemble
         auto_tiles(c, std::max(1, (c->ncu + n - 1) / n));
-        if ((r = ensure_partials(c))) return bail(r);
         b->ctlh.push_back(c->ctlh);
     }
     b->press = params[0].pressure ? 1 : 0;
+    batch_chunks(b);
+    for (gm_ctx *c : b->inst)
+        if ((r = ensure_partials(c))) return bail(r);
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(GM_ERR_HIP, "stream"));
     *out = b;
     return GM_OK;
