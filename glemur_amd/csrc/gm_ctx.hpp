@@ -45,6 +45,7 @@ struct gm_ctx {
     double *diag_scratch = nullptr;
     int diag_blocks = 2048;
     int *nanflag = nullptr;
+    double *xns_part = nullptr;   // per-kernel path: per-block error maxima of k_xnewstar
     int cur = 0;                  // host copy of ctl->cur (valid after syncs)
     // tiles
     // tiles per stage kernel (index 0..5 the stages, 6 the fields-only

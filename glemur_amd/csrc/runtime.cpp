@@ -427,7 +427,7 @@ extern "C" int gm_destroy(gm_ctx *c)
     dfree(c->B0_32); dfree(c->K32); dfree(c->B32); dfree(c->J32); dfree(c->G32);
     dfree(c->det); dfree(c->epsx); dfree(c->epsy); dfree(c->epsz);
     dfree(c->ctl); dfree(c->part_delta); dfree(c->part_diag); dfree(c->tab_dev); dfree(c->log); dfree(c->ghost);
-    dfree(c->red_out); dfree(c->diag_scratch); dfree(c->nanflag); dfree(c->extra_part);
+    dfree(c->red_out); dfree(c->diag_scratch); dfree(c->nanflag); dfree(c->extra_part); dfree(c->xns_part);
     if (c->ctlh) (void)hipHostFree(c->ctlh);
     if (c->red_host) (void)hipHostFree(c->red_host);
     for (auto e : c->ev) (void)hipEventDestroy(e);
@@ -711,10 +711,11 @@ extern "C" int gm_rk_xnewstar(gm_ctx *c, double *maxDelta)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     if (int r = need_f64(c, "gm_rk_xnewstar")) return r;
-    long long nb = (c->N + 255) / 256;
-    double *parts = nullptr;
-    int r = dalloc(&parts, (size_t)nb);
-    if (r) return r;
+    if (!c->xns_part) {   // one per-block maximum per 256 inner points, kept for later attempts
+        int r = dalloc(&c->xns_part, (size_t)((c->N + 255) / 256));
+        if (r) return r;
+    }
+    double *parts = c->xns_part;
     (void)hipMemsetAsync(c->nanflag, 0, sizeof(int), c->stream);
     double *xnew = c->X64[(c->cur + 1) % 3];
     // xb_new starts as xb (src/main.cu:209) so its ghost layers match
@@ -727,7 +728,6 @@ extern "C" int gm_rk_xnewstar(gm_ctx *c, double *maxDelta)
     gmd::launch_max_final(c->stream, parts, n, c->red_out);
     HIPCHK(hipMemcpyAsync(c->red_host, c->red_out, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    dfree(parts);
     if (maxDelta) *maxDelta = c->red_host[0];
     return GM_OK;
 }
