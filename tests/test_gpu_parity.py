@@ -370,3 +370,69 @@ def test_inertia_steps_bitwise(g, case):
     assert n_rej >= 1, "the trajectory should exercise the reject path"
     with pytest.raises(g.GlemurError):
         g.Context(p, g.PREC_F32K)
+
+
+@pytest.mark.parametrize("shape", [(2, 2, 2), (2, 3, 9), (7, 2, 3), (3, 5, 2)])
+def test_tiny_and_thin_grids(g, shape):
+    """Grids at the minimum of two inner points per axis and thin slabs,
+    where every point touches a line-tied face: 8 attempted steps of the
+    fused FP64 step (the oracle driven with the device's dt) and of the
+    per-kernel sequence on the same dt sequence, bit for bit with the oracle
+    (decisions, error estimates, positions); 8 F32K steps at a fixed dt far
+    inside the tolerance stay within 1e-5 of the displacement (plus 1e-8 of
+    the coordinates) of the oracle's positions.  (Adaptive F32K attempts within about 10% of
+    maxError may be decided differently: the error estimate cancels the
+    slopes by orders of magnitude and carries their FP32 rounding.)"""
+    p = small_braid(*shape)
+    B0, xb = evolved_state(p, 0)
+    sim = oracle_sim(p, B0, xb)
+    seq, n_rej = [], 0
+    with g.Context(p, g.PREC_F64) as a:
+        a.upload(B0, xb, 0.0, 0.01)
+        for it in range(8):
+            t, dt = a.time()
+            seq.append((t, dt))
+            sim.set_dt(dt)
+            sa = a.step()
+            acc, dtu, md, nan = sim.step()
+            assert not nan and bool(sa.accepted) == acc and sa.maxDelta == md and sa.dt_used == dtu, (shape, it)
+            n_rej += not acc
+        assert_bitwise(a.download(g.FIELD_XB), sim.arr("xb"), f"fused positions {shape}")
+    # F32K: 8 steps at dt = 0.01 (far inside the tolerance, all accepted in
+    # both precisions) against the oracle at the same dt
+    sim3 = oracle_sim(p, B0, xb)
+    with g.Context(p, g.PREC_F32K) as b:
+        b.upload(B0, xb, 0.0, 0.01)
+        for it in range(8):
+            b.set_time(b.time()[0], 0.01)
+            sim3.set_dt(0.01)
+            sb = b.step()
+            acc, _, md, _ = sim3.step()
+            assert sb.accepted and acc and md < 0.1 * p.maxError, (shape, it)
+        xf = b.download(g.FIELD_XB)
+    x3 = sim3.arr("xb")
+    # relative to the displacement, with a floor of 1e-8 of the coordinates
+    # (50 times below one FP32 ulp of them):
+    # on the 2 x 2 x 2 braid the grid moves by 3e-8 in these steps, and the
+    # FP32 Lorentz force there is a small difference of O(1) terms
+    dev, disp = np.max(np.abs(xf - x3)), np.max(np.abs(x3 - xb))
+    print(shape, f"F32K positions {dev:.2e} (displacement {disp:.2e})")
+    assert dev <= 1e-5 * disp + 1e-8 * np.max(np.abs(x3))
+    sim2 = oracle_sim(p, B0, xb)
+    with g.Context(p, g.PREC_F64) as c:
+        c.upload(B0, xb, 0.0, 0.01)
+        for it, (t, dt) in enumerate(seq):
+            c.set_time(t, dt)
+            sim2.set_dt(dt)
+            for n in range(6):
+                c.b_jacb0()
+                c.set_bbound()
+                c.current()
+                c.rk_kk(n, dt)
+            m = c.rk_xnewstar()
+            acc, _, md, _ = sim2.step()
+            assert m == md and (m < p.maxError) == acc, (shape, it)
+            if acc:
+                c.accept()
+        assert_bitwise(c.download(g.FIELD_XB), sim2.arr("xb"), f"per-kernel positions {shape}")
+    print(shape, f"{n_rej} of 8 attempts rejected")
