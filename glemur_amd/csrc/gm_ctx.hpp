@@ -47,6 +47,11 @@ struct gm_ctx {
     int *nanflag = nullptr;
     double *xns_part = nullptr;   // per-kernel path: per-block error maxima of k_xnewstar
     int cur = 0;                  // host copy of ctl->cur (valid after syncs)
+    // F32K: odd stages march z downwards, so each stage starts on the planes
+    // the previous one touched last, while they are in the 256 MB
+    // memory-side cache; on up to 16 M padded points (a stage moves up to
+    // about 1.3 GB there), off for larger grids and ensemble members
+    bool zrev = false;
     // tiles
     // tiles per stage kernel (index 0..5 the stages, 6 the fields-only
     // pass): the stages exchange data through HBM only, so each takes the

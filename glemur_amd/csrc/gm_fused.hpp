@@ -30,6 +30,7 @@ struct FusedArgs {
     int emit, diag, check_done;
     int face_table;                   // 1: every ghost face from the precomputed constants
                                       // (initial boundary distortion); 0: x/y faces direct (F32K)
+    int rev;                          // F32K: march z downwards (plane q stored at nz + 1 - q)
     // F64 buffers
     double *X64[3];                   // positions: current, candidate, stage-6 (3P each)
     const double *B0_64;

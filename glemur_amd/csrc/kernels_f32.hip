@@ -171,6 +171,12 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const int tx = Lb % A.tiles_x, ty = (Lb / A.tiles_x) % A.tiles_y, ch = Lb / (A.tiles_x * A.tiles_y);
     const int ix0 = tx * A.TX, iy0 = ty * A.TY;
     const int Ks = ch * A.CZ + 1, Ke = min(ch * A.CZ + A.CZ, g.nz);
+    // the march runs over virtual planes q; with A.rev it goes down the grid:
+    // q lives at physical plane nz + 1 - q, and z differences of the march
+    // order change sign (zs)
+    const bool rev = A.rev != 0;
+    auto zp = [&](int q) { return rev ? g.nz + 1 - q : q; };
+    const float zs = rev ? -1.f : 1.f;
     const int RW = A.TX + 4, RH = A.TY + 4;
     const int ixe = min(ix0 + A.TX, g.nx), iye = min(iy0 + A.TY, g.ny);
 
@@ -212,7 +218,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     auto ghost_pt = [&](int q) { return gx || gy || q == 0 || q == g.nz + 1; };
 
     auto load_plane = [&](int q, In &in) {
-        const int va = 4 * (int)(abase + g.sxy * (long long)q);
+        const int va = 4 * (int)(abase + g.sxy * (long long)zp(q));
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
             in.x[l] = bXh.ld(va, l * sP);
@@ -297,7 +303,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) drq[1][2][l] = dif(xzp, xzm, l);
+                for (int l = 0; l < 3; ++l) drq[1][2][l] = zs * dif(xzp, xzm, l);
                 // jac = dr diag(hx, hy, hz): det(jac) = det(dr) hx hy hz
                 // (det(dr) from its rows, the transpose of jac's columns),
                 // and jac B0 = dr (h * B0), so the spacing scales the three
@@ -331,7 +337,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     if (l == face) dn += h;
                     jac[l][face] = dn * h1n;
                     jac[l][tcol] = dif(ta, tb, l) * h1t;
-                    jac[l][2] = dif(xzp, xzm, l) * hz;
+                    jac[l][2] = zs * dif(xzp, xzm, l) * hz;
                 }
                 detv = cauchy32(jac, b0q[0], face, Bv);
             } else {
@@ -341,11 +347,11 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 double xin[3];
                 if (zg) {
                     face = 2;
-                    side = p != 0;
+                    side = zp(p) != 0;
                     u = ii;
                     v = jj;
                     Pt3 pin;
-                    if (side) L.ldx(p - 1, t, pin);
+                    if (p != 0) L.ldx(p - 1, t, pin);   // the inner neighbour in march order
                     else pin = xn;
 #pragma unroll
                     for (int l = 0; l < 3; ++l) xin[l] = (double)pin.h[l] + (double)pin.s[l];
@@ -353,7 +359,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     face = gx ? 0 : 1;
                     side = gx ? (gi != 0) : (gj != 0);
                     u = gx ? jj : ii;
-                    v = p - 1;
+                    v = zp(p) - 1;
                     const int st = face == 0 ? 1 : RW;
                     Pt3 pin;
                     L.ldx(p, side ? t - st : t + st, pin);
@@ -369,7 +375,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             if (A.emit && (gx || gy || zg)) {
                 int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
                 if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
-                    long long a = abase + g.sxy * (long long)p;
+                    long long a = abase + g.sxy * (long long)zp(p);
 #pragma unroll
                     for (int l = 0; l < 3; ++l) A.eB32[l * P + a] = Bv[l];
                     A.edet[a] = detv;
@@ -384,7 +390,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         // was published before the previous barrier, B(p) is this thread's own
         const int pj = p - 1;
         if (inJ && pj >= Ks && pj <= Ke) {
-            const long long a = abase + g.sxy * (long long)pj;
+            const long long a = abase + g.sxy * (long long)zp(pj);
             // Stokes current from central differences only.  For the
             // diamond through the face neighbours v0..v3 of the reference
             // (src/current.cu:291-382) the trapezoid circulation is
@@ -411,7 +417,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 L.ldb(pj + 1, t, bp);
                 L.ldb(pj - 1, t, bm);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) db[2][l] = bp[l] - bm[l];
+                for (int l = 0; l < 3; ++l) db[2][l] = zs * (bp[l] - bm[l]);
             }
             const float (&dr)[3][3] = drq[0];
             float Jv[3] = {0.f, 0.f, 0.f}, kv[3];
@@ -434,7 +440,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             kv[0] = Jv[1] * Bc[2] - Jv[2] * Bc[1];
             kv[1] = Jv[2] * Bc[0] - Jv[0] * Bc[2];
             kv[2] = Jv[0] * Bc[1] - Jv[1] * Bc[0];
-            const int kk = pj - 1;
+            const int kk = zp(pj) - 1;
             float eps = 1.f;
             if (g.epsilonProf) eps = eps_xy * (float)A.epsz[kk];
             float gv[3] = {0, 0, 0};
@@ -469,12 +475,12 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 L.ldx(pj + 1, t + 1, pa); L.ldx(pj - 1, t + 1, pb);
                 L.ldx(pj + 1, t - 1, pc); L.ldx(pj - 1, t - 1, pd);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d2[l][0][2] = d2[l][2][0] = (dif(pa, pb, l) - dif(pc, pd, l)) * h1[0] * h1[2] * 0.25f;
+                for (int l = 0; l < 3; ++l) d2[l][0][2] = d2[l][2][0] = zs * (dif(pa, pb, l) - dif(pc, pd, l)) * h1[0] * h1[2] * 0.25f;
                 // yz
                 L.ldx(pj + 1, t + RW, pa); L.ldx(pj - 1, t + RW, pb);
                 L.ldx(pj + 1, t - RW, pc); L.ldx(pj - 1, t - RW, pd);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d2[l][1][2] = d2[l][2][1] = (dif(pa, pb, l) - dif(pc, pd, l)) * h1[1] * h1[2] * 0.25f;
+                for (int l = 0; l < 3; ++l) d2[l][1][2] = d2[l][2][1] = zs * (dif(pa, pb, l) - dif(pc, pd, l)) * h1[1] * h1[2] * 0.25f;
                 gradp_core(jac, d2, __builtin_amdgcn_rcpf((float)detv), (float)g.beta, gv);
             }
 #pragma unroll

@@ -194,6 +194,7 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
     // layers off the lattice, where the reference's tile-edge rebuilding and
     // the ghost layer itself differ: those runs take every face from the table
     A.face_table = distorted_ghosts(c->p);
+    A.rev = c->zrev && stage >= 0 && (stage & 1);
     for (int i = 0; i < 3; ++i) A.X64[i] = c->X64[i];
     A.B0_64 = c->B0_64; A.K64 = c->K64;
     A.eB64 = c->B64; A.eJ64 = c->J64; A.eG64 = c->G64;
@@ -330,6 +331,7 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     (void)hipGetDevice(&c->device);
     c->geo = make_geo(c->p);
     c->P = c->geo.P;
+    c->zrev = precision == GM_PREC_F32K && c->P <= (16LL << 20);
     c->N = (long long)pp->nx * pp->ny * pp->nz;
     const size_t P3 = 3 * (size_t)c->P;
     int r = GM_OK;
@@ -1383,6 +1385,7 @@ extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm
         if ((r = gm_create(&params[i], precision, &c))) return bail(r);
         b->inst.push_back(c);
         c->rect_cap = 512;
+        c->zrev = false;   // the members together move several GB per stage
         auto_tiles(c, std::max(1, (c->ncu + n - 1) / n));
         b->ctlh.push_back(c->ctlh);
     }

@@ -35,6 +35,10 @@ TILE = (18, 16, 9)   # rects of 21..22 x 20 points (> 384): 512-thread workgroup
 
 
 def test_members_equal_standalone_runs(g):
+    """Each ensemble member equals its standalone run bit for bit.  The
+    standalone F32K contexts march every odd stage down the grid, the
+    members always upwards, so this also shows that the march direction
+    does not change a bit of the result."""
     ps = members(g)
     states = [g.init_state(p) for p in ps]
     with g.Batch(ps) as b:
