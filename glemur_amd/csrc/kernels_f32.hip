@@ -10,7 +10,14 @@
 //     reads instead of six;
 //   * a column's own z-neighbours are read back from the LDS rings (4
 //     position planes, 3 B planes) instead of being queued in registers, so
-//     every stage fits 128 VGPRs (two 512-thread workgroups per CU);
+//     without pressure stages 0-4 and the fields pass fit 128 VGPRs
+//     (1024-thread workgroups, tiles of about 26 x 26 columns) and the
+//     final stage 168 (768 threads); with pressure stages 0-4 run in 768-
+//     and the final stage and the fields pass in 512-thread workgroups;
+//   * on grids up to 16 M padded points odd stages march z downwards, so
+//     the planes the previous stage wrote last (still in the memory-side
+//     cache) are the first the next stage reads; reversal maps virtual
+//     plane q to nz + 1 - q and negates z differences, which is exact;
 //   * ghost faces x = const and y = const take their tangential Jacobian
 //     columns from the ghost layer itself (in LDS) and the outer ghost from
 //     x_ghost +- h n, with no table loads inside the march; the table of
