@@ -59,7 +59,6 @@ struct gm_ctx {
     int TX[7] = {}, TY[7] = {}, tiles_x[7] = {}, tiles_y[7] = {};
     int CZ[7] = {}, nchunks[7] = {}, nblocks[7] = {};
     int ncu = 256;
-    int rect_cap = 0;             // > 0: rects at most this large (ensemble members: 512)
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;

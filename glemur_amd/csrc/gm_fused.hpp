@@ -79,10 +79,13 @@ int blocks_per_cu32(int rect, int stage, int press);
 // Largest rect the FP32 kernel of this stage is built for (512, 768 or 1024).
 int max_rect32(int stage, int press);
 void launch_controller(hipStream_t s, const CtlArgs &A);
-// Ensemble launches: every workgroup of n instances in one launch (FP32
-// stage kernel, 512-thread rects), and one controller workgroup per instance.
-void launch_fused32_batch(hipStream_t s, int stage, int press, const FusedArgs *args, const int *start, int n,
-                          int total);
+// Ensemble launches: every workgroup of n instances in one launch of the
+// FP32 stage kernel in `nt`-thread workgroups (512, or 768 / 1024 where the
+// stage is built for them), and one controller workgroup per instance.
+// batch_threads32: the workgroup size a rect needs there (at least 512).
+int batch_threads32(int rect);
+void launch_fused32_batch(hipStream_t s, int stage, int press, int nt, const FusedArgs *args, const int *start,
+                          int n, int total);
 void launch_controller_batch(hipStream_t s, const CtlArgs *args, int n);
 
 }  // namespace gmd

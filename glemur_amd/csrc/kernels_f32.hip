@@ -725,37 +725,51 @@ int blocks_per_cu32(int rect, int stage, int press)
 void set_tableau32(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &t, sizeof t); }
 
 namespace {
-template <int STAGE, bool PRESS>
+template <int NT, int STAGE, bool PRESS>
 void launch_batch_one(hipStream_t s, const FusedArgs *args, const int *start, int n, int total)
 {
     static bool attr = false;
-    constexpr size_t lds = lds_bytes32<512>();
+    constexpr size_t lds = lds_bytes32<NT>();
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_stage32_batch<512, STAGE, PRESS>,
+        (void)hipFuncSetAttribute((const void *)k_stage32_batch<NT, STAGE, PRESS>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage32_batch<512, STAGE, PRESS>), dim3(total), dim3(512), lds, s, args, start, n);
+    hipLaunchKernelGGL((k_stage32_batch<NT, STAGE, PRESS>), dim3(total), dim3(NT), lds, s, args, start, n);
+}
+// The workgroup sizes a stage kernel is built for: 512 always, 768 and
+// 1024 where its registers fit (fits768, fits1024)
+template <int STAGE, bool PRESS>
+void launch_batch_nt(hipStream_t s, int nt, const FusedArgs *args, const int *start, int n, int total)
+{
+    if constexpr (fits1024(STAGE, PRESS))
+        if (nt == 1024) return launch_batch_one<1024, STAGE, PRESS>(s, args, start, n, total);
+    if constexpr (fits768(STAGE, PRESS))
+        if (nt == 768) return launch_batch_one<768, STAGE, PRESS>(s, args, start, n, total);
+    launch_batch_one<512, STAGE, PRESS>(s, args, start, n, total);
 }
 template <bool PRESS>
-void launch_batch_stage(hipStream_t s, int stage, const FusedArgs *args, const int *start, int n, int total)
+void launch_batch_stage(hipStream_t s, int stage, int nt, const FusedArgs *args, const int *start, int n,
+                        int total)
 {
     switch (stage) {
-    case 0: launch_batch_one<0, PRESS>(s, args, start, n, total); break;
-    case 1: launch_batch_one<1, PRESS>(s, args, start, n, total); break;
-    case 2: launch_batch_one<2, PRESS>(s, args, start, n, total); break;
-    case 3: launch_batch_one<3, PRESS>(s, args, start, n, total); break;
-    case 4: launch_batch_one<4, PRESS>(s, args, start, n, total); break;
-    default: launch_batch_one<5, PRESS>(s, args, start, n, total); break;
+    case 0: launch_batch_nt<0, PRESS>(s, nt, args, start, n, total); break;
+    case 1: launch_batch_nt<1, PRESS>(s, nt, args, start, n, total); break;
+    case 2: launch_batch_nt<2, PRESS>(s, nt, args, start, n, total); break;
+    case 3: launch_batch_nt<3, PRESS>(s, nt, args, start, n, total); break;
+    case 4: launch_batch_nt<4, PRESS>(s, nt, args, start, n, total); break;
+    default: launch_batch_nt<5, PRESS>(s, nt, args, start, n, total); break;
     }
 }
 }  // namespace
 
-void launch_fused32_batch(hipStream_t s, int stage, int press, const FusedArgs *args, const int *start, int n,
-                          int total)
+int batch_threads32(int rect) { return std::max(512, threads32(rect)); }
+
+void launch_fused32_batch(hipStream_t s, int stage, int press, int nt, const FusedArgs *args, const int *start,
+                          int n, int total)
 {
-    if (press) launch_batch_stage<true>(s, stage, args, start, n, total);
-    else launch_batch_stage<false>(s, stage, args, start, n, total);
+    if (press) launch_batch_stage<true>(s, stage, nt, args, start, n, total);
+    else launch_batch_stage<false>(s, stage, nt, args, start, n, total);
 }
 
 void launch_fused32(hipStream_t s, const FusedArgs &A)

@@ -104,8 +104,7 @@ int stage_blocks_per_cu(const gm_ctx *c, int k)
 int stage_max_rect(const gm_ctx *c, int k)
 {
     if (c->prec != GM_PREC_F32K) return gmd::fused_max_points(0);
-    const int m = gmd::max_rect32(k == 6 ? -1 : k, c->p.pressure);
-    return c->rect_cap > 0 ? std::min(m, c->rect_cap) : m;
+    return gmd::max_rect32(k == 6 ? -1 : k, c->p.pressure);
 }
 
 void set_tile_k(gm_ctx *c, int k, int tx, int ty, int cz)
@@ -1275,6 +1274,7 @@ struct gm_batch {
     gmd::FusedArgs *args[6] = {};   // per stage: one argument block per member
     int *start[6] = {};             // per stage: first workgroup of each member, then the total
     int total[6] = {};
+    int nt[6] = {};                 // per stage: workgroup size of the launch
     gmd::CtlArgs *cargs = nullptr;
     std::vector<gmd::Ctl *> ctlh;   // pinned host mirrors (the members' own)
     hipGraphExec_t gexec = nullptr;
@@ -1283,17 +1283,32 @@ struct gm_batch {
 
 namespace {
 
+// Workgroup size of a batch stage launch (k = 6: the fields pass, which
+// members run on their own): the largest any member's rect needs
+int batch_nt(const gm_batch *b, int k)
+{
+    int nt = 512;
+    for (const gm_ctx *c : b->inst) nt = std::max(nt, gmd::batch_threads32((c->TX[k] + 4) * (c->TY[k] + 4)));
+    return nt;
+}
+
 void batch_free_dev(gm_batch *b)
 {
     for (int s = 0; s < 6; ++s) { dfree(b->args[s]); dfree(b->start[s]); }
     dfree(b->cargs);
 }
 
-// Launch arguments from the members' current tiles and buffers.
+// Launch arguments from the members' current tiles and buffers; each
+// stage launches in workgroups as large as its largest member rect needs.
 int batch_prepare(gm_batch *b)
 {
     const int n = (int)b->inst.size();
     batch_free_dev(b);
+    for (int s = 0; s < 6; ++s) {
+        b->nt[s] = batch_nt(b, s);
+        if (b->nt[s] > gmd::max_rect32(s, b->press))
+            return fail(GM_ERR_ARG, "ensemble member tile exceeds the stage kernel's workgroup");
+    }
     std::vector<gmd::FusedArgs> A(n);
     std::vector<int> st(n + 1);
     std::vector<gmd::CtlArgs> C(n);
@@ -1334,7 +1349,8 @@ int batch_prepare(gm_batch *b)
 void batch_enqueue_step(gm_batch *b)
 {
     const int n = (int)b->inst.size();
-    for (int s = 0; s < 6; ++s) gmd::launch_fused32_batch(b->stream, s, b->press, b->args[s], b->start[s], n, b->total[s]);
+    for (int s = 0; s < 6; ++s)
+        gmd::launch_fused32_batch(b->stream, s, b->press, b->nt[s], b->args[s], b->start[s], n, b->total[s]);
     gmd::launch_controller_batch(b->stream, b->cargs, n);
 }
 
@@ -1348,8 +1364,7 @@ static void batch_chunks(gm_batch *b)
 {
     gm_ctx *c0 = b->inst[0];
     for (int k = 0; k < 7; ++k) {
-        // the batch stage kernel runs 512-thread workgroups
-        const long long slots = (long long)c0->ncu * gmd::blocks_per_cu32(512, k == 6 ? -1 : k, b->press);
+        const long long slots = (long long)c0->ncu * gmd::blocks_per_cu32(batch_nt(b, k), k == 6 ? -1 : k, b->press);
         int maxnz = 1;
         for (gm_ctx *c : b->inst) maxnz = std::max(maxnz, c->p.nz);
         int bestL = maxnz;
@@ -1384,7 +1399,6 @@ extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm
         gm_ctx *c = nullptr;
         if ((r = gm_create(&params[i], precision, &c))) return bail(r);
         b->inst.push_back(c);
-        c->rect_cap = 512;
         c->zrev = false;   // the members together move several GB per stage
         auto_tiles(c, std::max(1, (c->ncu + n - 1) / n));
         b->ctlh.push_back(c->ctlh);

@@ -94,3 +94,35 @@ def test_graph_replay_equals_stream(g):
             res.append([(m.time(), m.download(g.FIELD_XB)) for m in b.members])
     for (ta, xa), (tb, xb2) in zip(*res):
         assert ta == tb and np.array_equal(xa, xb2)
+
+
+def test_default_tiles_mix_workgroup_sizes(g):
+    """With their default tiles the members below need 1024-, 768- and
+    512-thread workgroups for stages 0-4; each stage launch runs at the
+    largest (members with smaller rects leave threads idle).  Every member
+    equals its standalone run with default tiles: the same tile per stage,
+    other z chunks, which change the order of the energy sum only."""
+    ps = [small_braid(52, 52, 24), small_braid(33, 30, 21), small_braid(19, 17, 23, epsilonProf=True)]
+    for p in ps:
+        p.nCycle = 10 ** 6
+    states = [g.init_state(p) for p in ps]
+    with g.Batch(ps) as b:
+        tiles = [[m.stage_tile(s)[:2] for s in range(6)] for m in b.members]
+        rects = [[(tx + 4) * (ty + 4) for tx, ty in t] for t in tiles]
+        assert max(rects[0][:5]) > 768 and 512 < max(rects[1][:5]) <= 768 and max(rects[2]) <= 512, rects
+        for m, (B0, xb) in zip(b.members, states):
+            m.upload(B0, xb, 0.0, 0.01)
+        stats = b.run(max_attempts=20, batch=5, use_graph=True)
+        got = [(m.download(g.FIELD_XB), m.time(), m.step_log(20)) for m in b.members]
+    for i, (p, (B0, xb)) in enumerate(zip(ps, states)):
+        with g.Context(p, g.PREC_F32K) as c:
+            assert [c.stage_tile(s)[:2] for s in range(6)] == tiles[i]
+            c.upload(B0, xb, 0.0, 0.01)
+            st = c.run(max_attempts=20, batch=5)
+            x, t, log = c.download(g.FIELD_XB), c.time(), c.step_log(20)
+        assert stats[i].n_attempt == st.n_attempt == 20 and stats[i].n_accept == st.n_accept
+        assert got[i][1] == t, i
+        assert np.array_equal(got[i][0], x), i
+        for r, q in zip(got[i][2], log):
+            assert (r.maxDelta, r.dt_used, r.accepted, r.ratio) == (q.maxDelta, q.dt_used, q.accepted, q.ratio), i
+            assert abs(r.energy - q.energy) <= 1e-12 * abs(q.energy), i
