@@ -135,13 +135,28 @@ int best_chunks(int nz, int cols, int slots)
     return best;
 }
 
+// 128-byte lines a launch loads per plane of one array for tiles of
+// (tlx x tly) rects of rw x rh points: a rect row is 4 rw contiguous bytes
+// at an arbitrary offset within a line, (4 rw + 124) / 128 lines on average
+double rect_lines(int tlx, int tly, int rw, int rh)
+{
+    return (double)tlx * tly * rh * (4.0 * rw + 124.0) / 128.0;
+}
+
 // Tile choice per stage kernel: among the rects (TX+4)(TY+4) that fit the
-// kernel's workgroup, the fewest loaded points per output point; then the z
-// chunks of best_chunks with `target` x the kernel's workgroups per CU
-// resident at a time.
+// kernel's workgroup,
+//  - F32K: the fewest 128-byte lines loaded per output point.  A load
+//    instruction costs about one request per line its lanes touch, so wide
+//    rects beat square ones with less halo: at 129 x 129 x 385, 65 x 10
+//    tiles take stages 0-4 4-7% faster than 26 x 26;
+//  - FP64: the fewest loaded points per output point (its register-queue
+//    march is 20% slower on the wide rects);
+// then the z chunks of best_chunks with `target` x the kernel's workgroups
+// per CU resident at a time.
 void auto_tiles(gm_ctx *c, int target)
 {
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const bool lines = c->prec == GM_PREC_F32K;
     for (int k = 0; k < 7; ++k) {
         const int cap = stage_max_rect(c, k);
         int bx = 4, by = 2;
@@ -151,8 +166,9 @@ void auto_tiles(gm_ctx *c, int target)
                 int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
                 int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
                 if ((tx + 4) * (tyy + 4) > cap) continue;
-                double cost = (double)tlx * tly * (tx + 4) * (tyy + 4) / ((double)nx * ny);
-                if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
+                const double cost = (lines ? rect_lines(tlx, tly, tx + 4, tyy + 4)
+                                           : (double)tlx * tly * (tx + 4) * (tyy + 4)) / ((double)nx * ny);
+                if (cost < best - 1e-12) { best = cost; bx = tx; by = tyy; }
             }
         set_tile_k(c, k, bx, by, nz);
         const int nch = best_chunks(nz, c->tiles_x[k] * c->tiles_y[k], target * stage_blocks_per_cu(c, k));
@@ -1172,7 +1188,9 @@ namespace {
 
 // Fewest loaded points per output point among the rects with
 // lo < (TX+4)(TY+4) <= hi.
-bool best_rect(const gm_ctx *c, int lo, int hi, int &bx, int &by)
+// The rect in (lo, hi] points with the fewest loaded points (lines = false)
+// or 128-byte lines (lines = true) per output point.
+bool best_rect(const gm_ctx *c, int lo, int hi, bool lines, int &bx, int &by)
 {
     const int nx = c->p.nx, ny = c->p.ny;
     double best = 1e30;
@@ -1182,8 +1200,8 @@ bool best_rect(const gm_ctx *c, int lo, int hi, int &bx, int &by)
             int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
             int rect = (tx + 4) * (tyy + 4);
             if (rect > hi || rect <= lo) continue;
-            double cost = (double)tlx * tly * rect / ((double)nx * ny);
-            if (cost < best - 1e-9) { best = cost; bx = tx; by = tyy; }
+            double cost = (lines ? rect_lines(tlx, tly, tx + 4, tyy + 4) : (double)tlx * tly * rect) / ((double)nx * ny);
+            if (cost < best - 1e-12) { best = cost; bx = tx; by = tyy; }
         }
     return best < 1e30;
 }
@@ -1212,10 +1230,20 @@ extern "C" int gm_autotune(gm_ctx *c, int reps, double ms_best[6])
         const int cap = stage_max_rect(c, k);
         int bTX = c->TX[k], bTY = c->TY[k], bCZ = c->CZ[k];
         double best = 1e30;
-        for (int w = 0; w < 5 && rc == GM_OK; ++w) {
+        // per workgroup size: the fewest-points rect and, for F32K, the
+        // fewest-lines one
+        std::vector<std::pair<int, int>> rects;
+        for (int w = 0; w < 5; ++w) {
             if (bounds[w] >= cap) break;
-            int tx, ty;
-            if (!best_rect(c, bounds[w], std::min(bounds[w + 1], cap), tx, ty)) continue;
+            for (int m = 0; m < (f32k ? 2 : 1); ++m) {
+                int tx, ty;
+                if (!best_rect(c, bounds[w], std::min(bounds[w + 1], cap), m == 1, tx, ty)) continue;
+                if (std::find(rects.begin(), rects.end(), std::make_pair(tx, ty)) == rects.end()) rects.emplace_back(tx, ty);
+            }
+        }
+        for (const auto &rt : rects) {
+            if (rc != GM_OK) break;
+            const int tx = rt.first, ty = rt.second;
             // z chunks: the plane-step model's choice, and one or two full
             // rounds over the CUs
             set_tile_k(c, k, tx, ty, c->p.nz);
