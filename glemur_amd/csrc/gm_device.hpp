@@ -105,9 +105,11 @@ void launch_aos_to_soa_f32k(hipStream_t s, long long P, const double *src, float
 void launch_soa_to_aos_f32k(hipStream_t s, long long P, const float *hi, const float *lo, double *dst);
 void launch_soa_to_aos_f32(hipStream_t s, long long P, const float *src, double *dst);
 void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float *dst);
-// Inner AoS (3N) <- padded SoA (3P), as J/gradP/kk are laid out on the host.
+// Inner AoS (3N) <- padded SoA (3P), as J/gradP/kk are laid out on the host;
+// triples: the source holds (x, y, z) of padded point a at 3a (the F32K
+// slopes)
 template <class R>
-void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, double *dst);
+void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, double *dst, bool triples = false);
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long bytes);
 void launch_fill_f64(hipStream_t s, double *p, long long n, double v);
 void launch_fill_f32(hipStream_t s, float *p, long long n, float v);

@@ -39,7 +39,7 @@ struct FusedArgs {
     // F32K buffers
     float *Xh[2], *Xl[2];
     const float *B0_32;
-    float *K32;                       // 5 x 3P
+    float *K32;                       // 5 x 3P: per slot, (x, y, z) of padded point a at 3a
     float *eB32, *eJ32, *eG32;
     double *edet;                     // emitted det (P)
     const double *epsx, *epsy, *epsz; // friction profile factors per axis

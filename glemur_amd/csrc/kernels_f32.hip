@@ -11,9 +11,13 @@
 //   * a column's own z-neighbours are read back from the LDS rings (4
 //     position planes, 3 B planes) instead of being queued in registers, so
 //     without pressure stages 0-4 and the fields pass fit 128 VGPRs
-//     (1024-thread workgroups, tiles of about 26 x 26 columns) and the
+//     (1024-thread workgroups, wide tiles such as 65 x 10 columns) and the
 //     final stage 168 (768 threads); with pressure stages 0-4 run in 768-
 //     and the final stage and the fields pass in 512-thread workgroups;
+//   * slopes are stored as (x, y, z) triples, one 12-byte load or store
+//     per point and slope: a rect row of a slope is one run of 12 (TX + 4)
+//     bytes, so its loads touch about a fifth fewer 128-byte lines than
+//     three component rows of 4 (TX + 4) bytes;
 //   * on grids up to 16 M padded points odd stages march z downwards, so
 //     the planes the previous stage wrote last (still in the memory-side
 //     cache) are the first the next stage reads; reversal maps virtual
@@ -95,6 +99,27 @@ struct BufF {
     __device__ __forceinline__ void st(float v, int voff, int soff) const
     {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    }
+    // three consecutive floats (x, y, z of one point) in one 12-byte access.
+    // The loaded vector is bit-cast whole: a bit_cast of one of its
+    // elements compiles to a single-dword load whose value lands in every
+    // component
+    __device__ __forceinline__ void ld3(int voff, float o[3]) const
+    {
+        typedef float v3f __attribute__((ext_vector_type(3)));
+        const v3f f = __builtin_bit_cast(v3f, __builtin_amdgcn_raw_buffer_load_b96(r, voff, 0, 0));
+        o[0] = f[0];
+        o[1] = f[1];
+        o[2] = f[2];
+    }
+    __device__ __forceinline__ void st3(const float o[3], int voff) const
+    {
+        typedef unsigned v3u __attribute__((ext_vector_type(3)));
+        v3u v;
+        v.x = __builtin_bit_cast(unsigned, o[0]);
+        v.y = __builtin_bit_cast(unsigned, o[1]);
+        v.z = __builtin_bit_cast(unsigned, o[2]);
+        __builtin_amdgcn_raw_buffer_store_b96(v, r, voff, 0, 0);
     }
 };
 // 0x00020000: the resource word-3 format of the gfx9 family
@@ -200,8 +225,9 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const long long abase = (long long)gi + g.sx * (long long)gj;
     const int ii = gi - 1, jj = gj - 1;
     const float eps_xy = (g.epsilonProf && inJ) ? (float)(A.epsx[ii] * A.epsy[jj]) : 1.0f;
-    // buffer views: positions, B0, one per slope slot; the component l of a
-    // point at padded index a sits at byte 4a (lane) + 4lP (scalar)
+    // buffer views: positions, B0, one per slope slot.  Positions and B0:
+    // the component l of a point at padded index a sits at byte 4a (lane) +
+    // 4lP (scalar); slopes: (x, y, z) of point a at bytes 12a .. 12a + 11
     const BufF bXh = buf_of(A.Xh[cur], P3), bXl = buf_of(A.Xl[cur], P3), bB0 = buf_of(A.B0_32, P3);
     BufF bK[6];
 #pragma unroll
@@ -235,9 +261,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         if (!ghost_pt(q)) {
 #pragma unroll
             for (int m = 0; m < 5; ++m)
-                if (m < n)
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) in.k[m][l] = bK[m].ld(va, l * sP);
+                if (m < n) bK[m].ld3(3 * va, in.k[m]);   // xyz triples at 12 a
         }
     };
 
@@ -528,9 +552,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             }
             if constexpr (!FINAL) {
                 if constexpr (stage >= 0) {
-                    const int va = 4 * (int)a;
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) bK[stage].st(kv[l], va, l * sP);
+                    bK[stage].st3(kv, 12 * (int)a);
                 }
             } else {
                 // x_new = x_5 + sum_m (c_m - b5_m) k_m (k_5 included), into

@@ -746,14 +746,14 @@ __global__ void k_aos_to_soa_f32(long long P, const double *src, float *dst)
     for (int l = 0; l < 3; ++l) dst[l * P + a] = (float)src[3 * a + l];
 }
 template <class R>
-__global__ void k_padded_to_inner_aos(Geo g, const R *src, double *dst)
+__global__ void k_padded_to_inner_aos(Geo g, const R *src, double *dst, bool triples)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long N = (long long)g.nx * g.ny * g.nz;
     if (t >= N) return;
     int i = (int)(t % g.nx), j = (int)((t / g.nx) % g.ny), k = (int)(t / ((long long)g.nx * g.ny));
     long long a = gidx(g, i + 1, j + 1, k + 1);
-    for (int l = 0; l < 3; ++l) dst[3 * t + l] = (double)src[l * g.P + a];
+    for (int l = 0; l < 3; ++l) dst[3 * t + l] = (double)(triples ? src[3 * a + l] : src[l * g.P + a]);
 }
 // Plain device copy: the reference kernel for HBM byte counters and the
 // achievable-bandwidth probe.
@@ -882,12 +882,12 @@ void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float 
     hipLaunchKernelGGL(k_aos_to_soa_f32, dim3(blocks_for(P)), dim3(kBlock), 0, s, P, src, dst);
 }
 template <class R>
-void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, double *dst)
+void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, double *dst, bool triples)
 {
-    hipLaunchKernelGGL(k_padded_to_inner_aos<R>, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, src, dst);
+    hipLaunchKernelGGL(k_padded_to_inner_aos<R>, dim3(blocks_for(ninner(g))), dim3(kBlock), 0, s, g, src, dst, triples);
 }
-template void launch_padded_to_inner_aos<double>(hipStream_t, const Geo &, const double *, double *);
-template void launch_padded_to_inner_aos<float>(hipStream_t, const Geo &, const float *, double *);
+template void launch_padded_to_inner_aos<double>(hipStream_t, const Geo &, const double *, double *, bool);
+template void launch_padded_to_inner_aos<float>(hipStream_t, const Geo &, const float *, double *, bool);
 void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long bytes)
 {
     long long n = bytes / 16;

@@ -639,7 +639,7 @@ extern "C" int gm_download(gm_ctx *c, int field, double *host)
         (void)hipMemsetAsync(tmp, 0, n * 8, s);
         for (int m = 0; m < (f64 ? 6 : 5); ++m) {
             if (f64) gmd::launch_padded_to_inner_aos<double>(s, c->geo, c->K64 + m * P3, tmp + m * N3);
-            else gmd::launch_padded_to_inner_aos<float>(s, c->geo, c->K32 + m * P3, tmp + m * N3);
+            else gmd::launch_padded_to_inner_aos<float>(s, c->geo, c->K32 + m * P3, tmp + m * N3, true);
         }
         break;
     }
