@@ -101,10 +101,13 @@ int launch_extra_diag(hipStream_t s, const Geo &g, const double *x64, const floa
 // dst[c*P + a] = src[3a + c] (src in double) with an optional hi/lo split.
 void launch_aos_to_soa_f64(hipStream_t s, long long P, const double *src, double *dst);
 void launch_soa_to_aos_f64(hipStream_t s, long long P, const double *src, double *dst);
+// F32K positions: host AoS <-> (hi, lo) triples (x, y, z of padded point a
+// at 3a); single FP32 arrays: component order, or triples (positions' lo
+// words, B0)
 void launch_aos_to_soa_f32k(hipStream_t s, long long P, const double *src, float *hi, float *lo);
 void launch_soa_to_aos_f32k(hipStream_t s, long long P, const float *hi, const float *lo, double *dst);
-void launch_soa_to_aos_f32(hipStream_t s, long long P, const float *src, double *dst);
-void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float *dst);
+void launch_soa_to_aos_f32(hipStream_t s, long long P, const float *src, double *dst, bool triples);
+void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float *dst, bool triples);
 // Inner AoS (3N) <- padded SoA (3P), as J/gradP/kk are laid out on the host;
 // triples: the source holds (x, y, z) of padded point a at 3a (the F32K
 // slopes)

@@ -19,7 +19,7 @@ struct PosRef {
     __device__ void at(long long a, double v[3]) const
     {
         for (int c = 0; c < 3; ++c)
-            v[c] = x64 ? x64[c * P + a] : (double)xh[c * P + a] + (double)xl[c * P + a];
+            v[c] = x64 ? x64[c * P + a] : (double)xh[3 * a + c] + (double)xl[3 * a + c];   // F32K: triples
     }
 };
 

@@ -14,10 +14,11 @@
 //     (1024-thread workgroups, wide tiles such as 65 x 10 columns) and the
 //     final stage 168 (768 threads); with pressure stages 0-4 run in 768-
 //     and the final stage and the fields pass in 512-thread workgroups;
-//   * slopes are stored as (x, y, z) triples, one 12-byte load or store
-//     per point and slope: a rect row of a slope is one run of 12 (TX + 4)
-//     bytes, so its loads touch about a fifth fewer 128-byte lines than
-//     three component rows of 4 (TX + 4) bytes;
+//   * positions (hi, lo), B0 and the slopes are stored as (x, y, z)
+//     triples, one 12-byte load or store per point and array: a rect row
+//     of an array is one run of 12 (TX + 4) bytes, so its loads touch about
+//     a fifth fewer 128-byte lines than three component rows of 4 (TX + 4)
+//     bytes;
 //   * on grids up to 16 M padded points odd stages march z downwards, so
 //     the planes the previous stage wrote last (still in the memory-side
 //     cache) are the first the next stage reads; reversal maps virtual
@@ -86,10 +87,9 @@ struct Lds {
 };
 
 // Raw buffer view of one float array: loads and stores take a 32-bit byte
-// offset per lane and a 32-bit byte offset held in a scalar register, so
-// the SoA component stride costs no per-lane 64-bit address arithmetic
-// (4% off every stage at 129 x 129 x 385); offsets past the array read 0
-// and drop writes.
+// offset per lane (and one held in a scalar register), so no per-lane
+// 64-bit address arithmetic runs (4% off every stage at 129 x 129 x 385);
+// offsets past the array read 0 and drop writes.
 struct BufF {
     __amdgpu_buffer_rsrc_t r;
     __device__ __forceinline__ float ld(int voff, int soff) const
@@ -225,14 +225,12 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const long long abase = (long long)gi + g.sx * (long long)gj;
     const int ii = gi - 1, jj = gj - 1;
     const float eps_xy = (g.epsilonProf && inJ) ? (float)(A.epsx[ii] * A.epsy[jj]) : 1.0f;
-    // buffer views: positions, B0, one per slope slot.  Positions and B0:
-    // the component l of a point at padded index a sits at byte 4a (lane) +
-    // 4lP (scalar); slopes: (x, y, z) of point a at bytes 12a .. 12a + 11
+    // buffer views: positions (hi, lo), B0, one per slope slot; each holds
+    // (x, y, z) of the point at padded index a at bytes 12a .. 12a + 11
     const BufF bXh = buf_of(A.Xh[cur], P3), bXl = buf_of(A.Xl[cur], P3), bB0 = buf_of(A.B0_32, P3);
     BufF bK[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) bK[m] = buf_of(A.K32 + (long long)(m < 5 ? m : 4) * P3, P3);
-    const int sP = (int)(4 * P);     // byte stride of a component
 
     // ---- registers carried between planes
     float b0q[2][3];      // B0 at planes p, p+1
@@ -252,12 +250,9 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 
     auto load_plane = [&](int q, In &in) {
         const int va = 4 * (int)(abase + g.sxy * (long long)zp(q));
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-            in.x[l] = bXh.ld(va, l * sP);
-            in.lo[l] = bXl.ld(va, l * sP);
-            in.b0[l] = bB0.ld(va, l * sP);
-        }
+        bXh.ld3(3 * va, in.x);
+        bXl.ld3(3 * va, in.lo);
+        bB0.ld3(3 * va, in.b0);
         if (!ghost_pt(q)) {
 #pragma unroll
             for (int m = 0; m < 5; ++m)
@@ -558,9 +553,10 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 // x_new = x_5 + sum_m (c_m - b5_m) k_m (k_5 included), into
                 // (hi, lo) by a two-sum; error |sum_m (c_m - c*_m) k_m| in FP64
                 const BufF bXhd = buf_of(A.Xh[cur ^ 1], P3), bXld = buf_of(A.Xl[cur ^ 1], P3);
-                const int va = 4 * (int)a;
+                const int va = 12 * (int)a;
                 Pt3 xc;
                 L.ldx(pj, t, xc);
+                float hn[3], ln[3];
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
                     float hi = xc.h[l], lo;
@@ -576,11 +572,13 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                         dmax = nb_max(dmax, dd);
                     } else {
                         // z held: x_5's small part is the old lo word (no slope terms on z)
-                        lo = bXl.ld(va, l * sP);
+                        lo = bXl.ld(va + 4 * l, 0);
                     }
-                    bXhd.st(hi, va, l * sP);
-                    bXld.st(lo, va, l * sP);
+                    hn[l] = hi;
+                    ln[l] = lo;
                 }
+                bXhd.st3(hn, va);
+                bXld.st3(ln, va);
             }
         }
 

@@ -73,7 +73,7 @@ __global__ void k_ghost_setup(FusedArgs A, GhostPt *out)
     auto pos = [&](long long q, double x[3]) {
         for (int l = 0; l < 3; ++l) {
             if (F32)
-                x[l] = (double)A.Xh[cur][l * g.P + q] + (double)A.Xl[cur][l * g.P + q];
+                x[l] = (double)A.Xh[cur][3 * q + l] + (double)A.Xl[cur][3 * q + l];   // triples
             else
                 x[l] = A.X64[cur][l * g.P + q];
         }
@@ -120,7 +120,7 @@ __global__ void k_ghost_setup(FusedArgs A, GhostPt *out)
         G.tc0[l] = jac[l][t0];
         G.tc1[l] = jac[l][t1];
         G.x2[l] = x2[l];
-        G.b0[l] = F32 ? (double)A.B0_32[l * g.P + a] : A.B0_64[l * g.P + a];
+        G.b0[l] = F32 ? (double)A.B0_32[3 * a + l] : A.B0_64[l * g.P + a];
     }
     out[A.ghost_off[fs] + u + (long long)nu * v] = G;
 }

@@ -716,6 +716,7 @@ __global__ void k_soa_to_aos_f64(long long P, const double *src, double *dst)
     if (a >= P) return;
     for (int l = 0; l < 3; ++l) dst[3 * a + l] = src[l * P + a];
 }
+// F32K positions are (hi, lo) triples: x, y, z of padded point a at 3a
 __global__ void k_aos_to_soa_f32k(long long P, const double *src, float *hi, float *lo)
 {
     long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -723,27 +724,28 @@ __global__ void k_aos_to_soa_f32k(long long P, const double *src, float *hi, flo
     for (int l = 0; l < 3; ++l) {
         double v = src[3 * a + l];
         float h = (float)v;
-        hi[l * P + a] = h;
-        lo[l * P + a] = (float)(v - (double)h);
+        hi[3 * a + l] = h;
+        lo[3 * a + l] = (float)(v - (double)h);
     }
 }
 __global__ void k_soa_to_aos_f32k(long long P, const float *hi, const float *lo, double *dst)
 {
     long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P) return;
-    for (int l = 0; l < 3; ++l) dst[3 * a + l] = (double)hi[l * P + a] + (double)lo[l * P + a];
+    for (int l = 0; l < 3; ++l) dst[3 * a + l] = (double)hi[3 * a + l] + (double)lo[3 * a + l];
 }
-__global__ void k_soa_to_aos_f32(long long P, const float *src, double *dst)
+// FP32 arrays in component order (triples = false) or as triples
+__global__ void k_soa_to_aos_f32(long long P, const float *src, double *dst, bool triples)
 {
     long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P) return;
-    for (int l = 0; l < 3; ++l) dst[3 * a + l] = (double)src[l * P + a];
+    for (int l = 0; l < 3; ++l) dst[3 * a + l] = (double)(triples ? src[3 * a + l] : src[l * P + a]);
 }
-__global__ void k_aos_to_soa_f32(long long P, const double *src, float *dst)
+__global__ void k_aos_to_soa_f32(long long P, const double *src, float *dst, bool triples)
 {
     long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P) return;
-    for (int l = 0; l < 3; ++l) dst[l * P + a] = (float)src[3 * a + l];
+    for (int l = 0; l < 3; ++l) dst[triples ? 3 * a + l : l * P + a] = (float)src[3 * a + l];
 }
 template <class R>
 __global__ void k_padded_to_inner_aos(Geo g, const R *src, double *dst, bool triples)
@@ -873,13 +875,13 @@ void launch_soa_to_aos_f32k(hipStream_t s, long long P, const float *hi, const f
 {
     hipLaunchKernelGGL(k_soa_to_aos_f32k, dim3(blocks_for(P)), dim3(kBlock), 0, s, P, hi, lo, dst);
 }
-void launch_soa_to_aos_f32(hipStream_t s, long long P, const float *src, double *dst)
+void launch_soa_to_aos_f32(hipStream_t s, long long P, const float *src, double *dst, bool triples)
 {
-    hipLaunchKernelGGL(k_soa_to_aos_f32, dim3(blocks_for(P)), dim3(kBlock), 0, s, P, src, dst);
+    hipLaunchKernelGGL(k_soa_to_aos_f32, dim3(blocks_for(P)), dim3(kBlock), 0, s, P, src, dst, triples);
 }
-void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float *dst)
+void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float *dst, bool triples)
 {
-    hipLaunchKernelGGL(k_aos_to_soa_f32, dim3(blocks_for(P)), dim3(kBlock), 0, s, P, src, dst);
+    hipLaunchKernelGGL(k_aos_to_soa_f32, dim3(blocks_for(P)), dim3(kBlock), 0, s, P, src, dst, triples);
 }
 template <class R>
 void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, double *dst, bool triples)

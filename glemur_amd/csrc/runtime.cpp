@@ -473,8 +473,10 @@ extern "C" int gm_upload(gm_ctx *c, const double *B0, const double *xb, double t
         // B starts as a copy of B0 (src/main.cu:195); ghost edges keep it
         HIPCHK(hipMemcpyAsync(c->B64, c->B0_64, P3 * 8, hipMemcpyDeviceToDevice, s));
     } else {
-        gmd::launch_aos_to_soa_f32(s, c->P, stage, c->B0_32);
-        HIPCHK(hipMemcpyAsync(c->B32, c->B0_32, P3 * 4, hipMemcpyDeviceToDevice, s));
+        // B0 as triples for the stage kernels; B (component order) starts as
+        // a copy of it
+        gmd::launch_aos_to_soa_f32(s, c->P, stage, c->B0_32, true);
+        gmd::launch_aos_to_soa_f32(s, c->P, stage, c->B32, false);
     }
     if (c->p.inertia) {
         // the velocity starts at rest (uInit = 'nil'); gm_upload_velocity
@@ -572,13 +574,13 @@ extern "C" int gm_download(gm_ctx *c, int field, double *host)
             else gmd::launch_soa_to_aos_f32k(s, c->P, c->Xh[idx], c->Xl[idx], tmp);
         } else if (field == GM_FIELD_XB_LO) {
             if (f64) (void)hipMemsetAsync(tmp, 0, n * 8, s);
-            else gmd::launch_soa_to_aos_f32(s, c->P, c->Xl[idx], tmp);
+            else gmd::launch_soa_to_aos_f32(s, c->P, c->Xl[idx], tmp, true);
         } else if (field == GM_FIELD_B0) {
             if (f64) gmd::launch_soa_to_aos_f64(s, c->P, c->B0_64, tmp);
-            else gmd::launch_soa_to_aos_f32(s, c->P, c->B0_32, tmp);
+            else gmd::launch_soa_to_aos_f32(s, c->P, c->B0_32, tmp, true);
         } else {
             if (f64) gmd::launch_soa_to_aos_f64(s, c->P, c->B64, tmp);
-            else gmd::launch_soa_to_aos_f32(s, c->P, c->B32, tmp);
+            else gmd::launch_soa_to_aos_f32(s, c->P, c->B32, tmp, false);
         }
         break;
     }

@@ -420,7 +420,7 @@ __global__ void k_features(const double *B64, const float *B32, long long Pf, in
     const long long gi = (stride * i + 1) + (long long)fsx * (stride * j + 1) + fsxy * (stride * k + 1);
     float f[8];
     for (int c = 0; c < 3; ++c) {
-        const double b = B64 ? B64[c * Pf + gi] : (double)B32[c * Pf + gi];
+        const double b = B64 ? B64[c * Pf + gi] : (double)B32[3 * gi + c];   // FP32 B0: triples
         f[c] = (float)(c == 2 ? b - 1.0 : b);
     }
     f[3] = (float)(2.0 * i / (nx - 1) - 1.0);
@@ -441,7 +441,7 @@ __global__ void k_target(const double *X64, const float *Xh, const float *Xl, lo
     const long long gi = (i + 1) + (long long)fsx * (j + 1) + fsxy * (k + 1);
     const double o[3] = {Ox + i * dx, Oy + j * dy, Oz + k * dz};
     for (int c = 0; c < 3; ++c) {
-        const double x = X64 ? X64[c * Pf + gi] : (double)Xh[c * Pf + gi] + (double)Xl[c * Pf + gi];
+        const double x = X64 ? X64[c * Pf + gi] : (double)Xh[3 * gi + c] + (double)Xl[3 * gi + c];
         target[q * 3 + c] = (float)(x - o[c]);
     }
 }
@@ -479,10 +479,10 @@ __global__ void k_apply(const float *y, int cnx, int cny, int cnz, int stride, d
         if (X64) {
             X64[c * Pf + gi] += dc;
         } else {
-            const double x = (double)Xh[c * Pf + gi] + (double)Xl[c * Pf + gi] + dc;
+            const double x = (double)Xh[3 * gi + c] + (double)Xl[3 * gi + c] + dc;   // F32K: triples
             const float h = (float)x;
-            Xh[c * Pf + gi] = h;
-            Xl[c * Pf + gi] = (float)(x - (double)h);
+            Xh[3 * gi + c] = h;
+            Xl[3 * gi + c] = (float)(x - (double)h);
         }
     }
     (void)cnz;
