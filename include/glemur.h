@@ -7,7 +7,8 @@
  *   init -> boundaries -> current -> gradP -> rungeKutta -> diagnostics
  * and cites the reference function it stands in for.  Host arrays use the
  * reference's own layout (AoS, padded (nx+2)(ny+2)(nz+2) points, x fastest);
- * the library keeps structure-of-arrays copies on the device.
+ * the library keeps its own device copies (FP64: one array per component;
+ * FP32 with compensated positions: padded (x, y, z) triples).
  *
  * Every function returns GM_OK (0) or a negative GM_ERR_* code; the text of
  * the last error of the calling thread is available from gm_last_error().
