@@ -151,14 +151,21 @@ double rect_lines(int tlx, int tly, int rw, int rh)
 //    tiles take stages 0-4 4-7% faster than 26 x 26;
 //  - FP64: the fewest loaded points per output point (its register-queue
 //    march is 20% slower on the wide rects);
+//  - F32K stage 0 without pressure, outside ensembles: the fewest loaded
+//    points among rects of up to 512 points, two workgroups per CU.  It
+//    waits on its barriers more than any other stage, and two independent
+//    workgroups per CU hide that: at 129 x 129 x 385 on 26 x 13 tiles it
+//    takes 134 instead of 146 us on 65 x 10 (ensemble members lose 1%
+//    with it, on their smaller grids, and keep the line model);
 // then the z chunks of best_chunks with `target` x the kernel's workgroups
 // per CU resident at a time.
-void auto_tiles(gm_ctx *c, int target)
+void auto_tiles(gm_ctx *c, int target, bool member)
 {
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     const bool lines = c->prec == GM_PREC_F32K;
     for (int k = 0; k < 7; ++k) {
-        const int cap = stage_max_rect(c, k);
+        const bool two = lines && k == 0 && !c->p.pressure && !member;
+        const int cap = two ? 512 : stage_max_rect(c, k);
         int bx = 4, by = 2;
         double best = 1e30;
         for (int ty = 2; ty <= 96; ++ty)
@@ -166,7 +173,7 @@ void auto_tiles(gm_ctx *c, int target)
                 int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
                 int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
                 if ((tx + 4) * (tyy + 4) > cap) continue;
-                const double cost = (lines ? rect_lines(tlx, tly, tx + 4, tyy + 4)
+                const double cost = (lines && !two ? rect_lines(tlx, tly, tx + 4, tyy + 4)
                                            : (double)tlx * tly * (tx + 4) * (tyy + 4)) / ((double)nx * ny);
                 if (cost < best - 1e-12) { best = cost; bx = tx; by = tyy; }
             }
@@ -425,7 +432,7 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
         hipDeviceProp_t pr;
         c->ncu = hipGetDeviceProperties(&pr, c->device) == hipSuccess ? pr.multiProcessorCount : 256;
     }
-    auto_tiles(c, default_blocks(c));
+    auto_tiles(c, default_blocks(c), false);
     if ((r = ensure_partials(c))) return bail(r);
     if ((r = push_ctl(c))) return bail(r);
     *out = c;
@@ -1099,7 +1106,7 @@ extern "C" int gm_set_tile(gm_ctx *c, int tx, int ty, int cz)
     if (!c) return fail(GM_ERR_ARG, "null context");
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (tx <= 0 || ty <= 0) {
-        auto_tiles(c, cz > 0 ? cz : default_blocks(c));
+        auto_tiles(c, cz > 0 ? cz : default_blocks(c), false);
         return ensure_partials(c);
     }
     if ((tx + 4) * (ty + 4) > gmd::fused_max_points(c->prec == GM_PREC_F32K))
@@ -1430,7 +1437,7 @@ extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm
         if ((r = gm_create(&params[i], precision, &c))) return bail(r);
         b->inst.push_back(c);
         c->zrev = false;   // the members together move several GB per stage
-        auto_tiles(c, std::max(1, (c->ncu + n - 1) / n));
+        auto_tiles(c, std::max(1, (c->ncu + n - 1) / n), true);
         b->ctlh.push_back(c->ctlh);
     }
     b->press = params[0].pressure ? 1 : 0;
