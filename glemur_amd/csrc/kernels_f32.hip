@@ -42,6 +42,16 @@ __constant__ TableauF c_tab32;
 
 namespace {
 
+// Plane steps between a plane's loads and its use: two for the stage
+// kernels whose registers hold a second input slot without spilling (bit s
+// of the mask: stage s, 6 the fields pass; pressure variants take one)
+#ifndef GM_PF2_STAGES
+#define GM_PF2_STAGES 0x4f
+#endif
+#ifndef GM_PF2_STAGES_P
+#define GM_PF2_STAGES_P 0x1f
+#endif
+constexpr int pf_depth(int stage, bool press) { return ((press ? GM_PF2_STAGES_P : GM_PF2_STAGES) >> stage) & 1 ? 2 : 1; }
 constexpr int kNXL = 4;      // position planes in LDS
 constexpr int kNBL = 3;      // B planes in LDS
 
@@ -53,6 +63,9 @@ __device__ __forceinline__ float dif(const Pt3 &a, const Pt3 &b, int l)
 {
     return (a.h[l] - b.h[l]) + (a.s[l] - b.s[l]);
 }
+
+// NaN-blind running maximum (a NaN candidate leaves m unchanged)
+__device__ __forceinline__ float nb_maxf(float m, float v) { return v > m ? v : m; }
 
 template <int NT>
 struct Lds {
@@ -122,6 +135,10 @@ struct BufF {
         __builtin_amdgcn_raw_buffer_store_b96(v, r, voff, 0, 0);
     }
 };
+// Store offset of lanes without an output point: 2^31 bytes lies past
+// every array (an F32K array holds fewer than 2^31 bytes), so the buffer
+// range check drops the store
+constexpr int kNoStore = (int)0x80000000u;
 // 0x00020000: the resource word-3 format of the gfx9 family
 __device__ __forceinline__ BufF buf_of(const float *p, long long nfloats)
 {
@@ -184,6 +201,10 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     constexpr int stage = STAGE == 6 ? -1 : STAGE;
     constexpr int n = STAGE == 6 ? 0 : STAGE;   // slopes entering the stage position
     constexpr bool DIAG = STAGE == 0;            // diagnostics ride on stage 0 only
+    // only the fields pass writes B, J, det, gradP; the pressure kernels keep
+    // the (runtime-off) branch: without it their stage 0 takes 30 more VGPRs
+    constexpr bool EMIT = STAGE == 6 || PRESS;
+    constexpr int kPF = pf_depth(STAGE, PRESS);
 
     extern __shared__ __align__(16) unsigned char smem[];
     Lds<NT> L;
@@ -218,17 +239,34 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
     const bool col = t < RW * RH && gi >= 0 && gi <= g.nx + 1 && gj >= 0 && gj <= g.ny + 1;
     const bool gx = gi == 0 || gi == g.nx + 1, gy = gj == 0 || gj == g.ny + 1;
+    // virtual columns one step beyond a line-tied x or y face hold the outer
+    // ghost x_g -+ h n of the face point next to them (src/boundaries.cu:
+    // 164-186, 291-313), so the face point's Jacobian is the interior
+    // central-difference formula and no wave runs a separate face branch
+    const int vx = t < RW * RH && gj >= 0 && gj <= g.ny + 1 ? (gi == -1 ? -1 : gi == g.nx + 2 ? 1 : 0) : 0;
+    const int vy = t < RW * RH && gi >= 0 && gi <= g.nx + 1 ? (gj == -1 ? -1 : gj == g.ny + 2 ? 1 : 0) : 0;
+    const bool vcol = !A.face_table && (vx != 0 || vy != 0);
+    const bool ldc = col || vcol;       // columns whose stage positions go to LDS
     const bool inB = col && c >= 1 && c <= RW - 2 && r >= 1 && r <= RH - 2 &&
                      !((c == 1 || c == RW - 2) && (r == 1 || r == RH - 2)) && !(gx && gy);
     const bool inJ = col && c >= 2 && c <= RW - 3 && r >= 2 && r <= RH - 3 && gi >= 1 && gi <= ixe && gj >= 1 &&
                      gj <= iye;
-    const long long abase = (long long)gi + g.sx * (long long)gj;
+    // the padded point a lane loads: a virtual column's face point; lanes
+    // past the rect repeat its last point and lanes outside the grid the
+    // nearest padded point, so every lane's loads are in bounds and a step's
+    // loads are issued by every lane (unconditionally: the compiler then
+    // counts them and waits only for the slot a step consumes)
+    const int tl = min(t, RW * RH - 1);
+    const int gil = ix0 - 1 + tl % RW, gjl = iy0 - 1 + tl / RW;
+    const long long abase = (long long)min(max(gil, 0), g.nx + 1) + g.sx * (long long)min(max(gjl, 0), g.ny + 1);
     const int ii = gi - 1, jj = gj - 1;
     const float eps_xy = (g.epsilonProf && inJ) ? (float)(A.epsx[ii] * A.epsy[jj]) : 1.0f;
     // buffer views: positions (hi, lo), B0, one per slope slot; each holds
     // (x, y, z) of the point at padded index a at bytes 12a .. 12a + 11
     const BufF bXh = buf_of(A.Xh[cur], P3), bXl = buf_of(A.Xl[cur], P3), bB0 = buf_of(A.B0_32, P3);
     BufF bK[6];
+    const BufF bXhd = buf_of(A.Xh[cur ^ 1], P3), bXld = buf_of(A.Xl[cur ^ 1], P3);   // final stage: x_new
+    (void)bXhd; (void)bXld;
 #pragma unroll
     for (int m = 0; m < 6; ++m) bK[m] = buf_of(A.K32 + (long long)(m < 5 ? m : 4) * P3, P3);
 
@@ -238,48 +276,50 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     float drq[2][3][3];   // central differences x(+a) - x(-a) at planes p-1, p
     struct In {
         float x[3], lo[3], k[5][3], b0[3];
-    } pf;                 // inputs of one plane, loaded one iteration ahead
+    } pf0, pf1;           // inputs of one plane, loaded kPF plane steps ahead
     float fg[3][3], fe[3][3];   // final stage: per plane sum (c - b5) k and sum (c - c*) k
     (void)fg; (void)fe;
 
-    double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
+    double dmax = 0, e_sum = 0;
+    float jmax2 = 0.f, r1q = 0.f, r2q = 0.f, r3q = 0.f, r4q = 0.f;   // stage-0 maxima, squared
     int nan_seen = 0;
 
     auto plane_ok = [&](int q) { return q >= max(Ks - 2, 0) && q <= min(Ke + 2, g.nz + 1); };
-    auto ghost_pt = [&](int q) { return gx || gy || q == 0 || q == g.nz + 1; };
 
+    // Every lane loads every step; planes outside the chunk's range repeat
+    // its first or last plane (cache hits, no extra HBM traffic).  Slopes of
+    // ghost points are zero in memory (never written), so a ghost point's
+    // stage position needs no test
     auto load_plane = [&](int q, In &in) {
-        const int va = 4 * (int)(abase + g.sxy * (long long)zp(q));
+        const int qc = min(max(q, max(Ks - 2, 0)), min(Ke + 2, g.nz + 1));
+        const int va = 4 * (int)(abase + g.sxy * (long long)zp(qc));
         bXh.ld3(3 * va, in.x);
         bXl.ld3(3 * va, in.lo);
         bB0.ld3(3 * va, in.b0);
-        if (!ghost_pt(q)) {
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                if (m < n) bK[m].ld3(3 * va, in.k[m]);   // xyz triples at 12 a
-        }
+        for (int m = 0; m < 5; ++m)
+            if (m < n) bK[m].ld3(3 * va, in.k[m]);   // xyz triples at 12 a
     };
 
-    // ---- march
-    const int p_first = Ks - 4, p_last = Ke + 1;
-    for (int p = p_first; p <= p_last; ++p) {
-        // (1) stage position of plane p+1 from the inputs loaded last iteration
+    // ---- march.  Plane p+1+kPF is loaded at step p into one of kPF input
+    // slots and turned into a stage position kPF steps later; with two
+    // slots the loop is unrolled by two so that each step names its slot
+    // statically (no register indexing)
+    auto plane_step = [&](const int p, In &pf) {
+        // (1) stage position of plane p+1 from the inputs loaded kPF iterations ago
         Pt3 xn;
-        if (col && plane_ok(p + 1)) {
-            const bool gh = ghost_pt(p + 1);
+        if (ldc && plane_ok(p + 1)) {
 #pragma unroll
             for (int l = 0; l < 3; ++l) {
                 float s = pf.lo[l];
-                if (!gh)
 #pragma unroll
-                    for (int m = 0; m < 5; ++m)
-                        if (m < n) s += (float)c_tab32.b[n * 6 + m] * pf.k[m][l];
+                for (int m = 0; m < 5; ++m)
+                    if (m < n) s += (float)c_tab32.b[n * 6 + m] * pf.k[m][l];
                 xn.h[l] = pf.x[l];
                 xn.s[l] = s;
                 b0q[1][l] = pf.b0[l];
                 if constexpr (FINAL) {
                     float a1 = 0, a2 = 0;
-                    if (!gh)
 #pragma unroll
                         for (int m = 0; m < 5; ++m) {
                             a1 += (float)(c_tab32.c[m] - c_tab32.b[5 * 6 + m]) * pf.k[m][l];
@@ -289,10 +329,20 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     fe[2][l] = a2;
                 }
             }
+            if (vcol) {
+                // outer ghost: the face point shifted by h along the face
+                // normal, the shift in the small part so that the face
+                // point's difference (h_in - h_face) + (s_in - s_face -+ h)
+                // keeps the high words' exact difference
+                if (vx) xn.s[0] += (float)vx * (float)g.dx;
+                if (vy) xn.s[1] += (float)vy * (float)g.dy;
+            }
             L.stx(p + 1, t, xn);
         }
-        // (0) loads of plane p+2, in flight while B and J are computed
-        if (col && plane_ok(p + 2)) load_plane(p + 2, pf);
+        // (0) loads of plane p+1+kPF into the slot just read, in flight
+        // while B and J are computed (and, with kPF = 2, while the next
+        // plane step runs on the other slot)
+        load_plane(p + 1 + kPF, pf);
         __builtin_amdgcn_sched_barrier(0);
 
         // (2) B at (column, p); ghost edges keep B0 and are never read by J
@@ -307,10 +357,13 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 xzp = xn;                 // own plane p+1, formed above
                 L.ldx(p - 1, t, xzm);
             }
-            if (!gx && !gy && !zg) {
+            if (!zg && !(A.face_table && (gx || gy))) {
                 // central differences of x along each axis, one neighbour
                 // pair at a time; they give the Jacobian here and, kept for
-                // one plane, the current at this point in the next iteration
+                // one plane, the current at this point in the next iteration.
+                // On a line-tied x or y face the neighbour beyond the face
+                // is the outer ghost x_g -+ h n (a virtual column), the
+                // tangential ones the ghost layer itself, and B . n = B0 . n
                 const float hx = 0.5f * (float)g.dx1, hy = 0.5f * (float)g.dy1, hz = 0.5f * (float)g.dz1;
                 {
                     Pt3 a, b;
@@ -340,32 +393,8 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 #pragma unroll
                 for (int l = 0; l < 3; ++l)
                     Bv[l] = (drq[1][0][l] * bs[0] + drq[1][1][l] * bs[1] + drq[1][2][l] * bs[2]) * rd;
-            } else if (!zg && !A.face_table) {
-                // x or y face, line-tied: tangential columns from the ghost
-                // layer, normal column from the inner neighbour and the
-                // outer ghost x_g +- h n (src/boundaries.cu:164-186, 291-313)
-                const int face = gx ? 0 : 1;
-                const int side = gx ? (gi != 0) : (gj != 0);
-                const int st = face == 0 ? 1 : RW;
-                Pt3 xin, ta, tb;
-                L.ldx(p, side ? t - st : t + st, xin);
-                L.ldx(p, t + (face == 0 ? RW : 1), ta);   // + along the in-plane tangent
-                L.ldx(p, t - (face == 0 ? RW : 1), tb);
-                const float h = (float)(face == 0 ? g.dx : g.dy);
-                const float h1n = 0.5f * (float)(face == 0 ? g.dx1 : g.dy1);
-                const float h1t = 0.5f * (float)(face == 0 ? g.dy1 : g.dx1);
-                const float hz = 0.5f * (float)g.dz1;
-                const int tcol = face == 0 ? 1 : 0;
-                float jac[3][3];
-#pragma unroll
-                for (int l = 0; l < 3; ++l) {
-                    float dn = side ? dif(xc, xin, l) : dif(xin, xc, l);
-                    if (l == face) dn += h;
-                    jac[l][face] = dn * h1n;
-                    jac[l][tcol] = dif(ta, tb, l) * h1t;
-                    jac[l][2] = zs * dif(xzp, xzm, l) * hz;
-                }
-                detv = cauchy32(jac, b0q[0], face, Bv);
+                if (gx) Bv[0] = b0q[0][0];
+                if (gy) Bv[1] = b0q[0][1];
             } else {
                 // z faces (and every face of a grid with an initial boundary
                 // distortion): precomputed face constants
@@ -398,7 +427,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 #pragma unroll
                 for (int l = 0; l < 3; ++l) Bv[l] = (float)Bd[l];
             }
-            if (A.emit && (gx || gy || zg)) {
+            if (EMIT && A.emit && (gx || gy || zg)) {
                 int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
                 if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
                     long long a = abase + g.sxy * (long long)zp(p);
@@ -415,8 +444,15 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         // (3) J, force and slope at (column, p-1): in-plane data of plane p-1
         // was published before the previous barrier, B(p) is this thread's own
         const int pj = p - 1;
+        // this step's stores are issued by every lane: lanes without an
+        // output point store at byte offset 2^31, past every array, where
+        // the buffer store is dropped
+        int st_off = kNoStore;
+        float kst[3] = {0.f, 0.f, 0.f}, hn[3], ln[3];
+        (void)kst; (void)hn; (void)ln;
         if (inJ && pj >= Ks && pj <= Ke) {
             const long long a = abase + g.sxy * (long long)zp(pj);
+            st_off = 12 * (int)a;
             // Stokes current from central differences only.  For the
             // diamond through the face neighbours v0..v3 of the reference
             // (src/current.cu:291-382) the trapezoid circulation is
@@ -514,7 +550,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 kv[l] *= dt * eps;
                 if (PRESS) kv[l] += -dt * gv[l];
             }
-            if (A.emit) {
+            if (EMIT && A.emit) {
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
                     A.eB32[l * P + a] = Bc[l];
@@ -524,39 +560,35 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 A.edet[a] = detv;
             }
             if (DIAG && A.diag) {
-                // squared magnitudes in FP64 (products of floats, exact but
-                // for the sums); the maxima are kept squared and rooted once
-                // per thread after the march; the two per-point quotients
-                // take the hardware reciprocal in FP32
-                double Bd[3] = {(double)Bc[0], (double)Bc[1], (double)Bc[2]};
-                double Jd[3] = {(double)Jv[0], (double)Jv[1], (double)Jv[2]}, jxb[3];
-                cross3(Jd, Bd, jxb);
-                const double b2 = Bd[0] * Bd[0] + Bd[1] * Bd[1] + Bd[2] * Bd[2];
-                const double nc2 = jxb[0] * jxb[0] + jxb[1] * jxb[1] + jxb[2] * jxb[2];
-                const double nj2 = Jd[0] * Jd[0] + Jd[1] * Jd[1] + Jd[2] * Jd[2];
-                const double den2 = nj2 * b2;
+                // squared magnitudes in FP32, maxima kept squared and rooted
+                // once per thread after the march, per-point quotients by
+                // the hardware reciprocal; only the energy sum runs in FP64:
+                // |B|^2 / 2^bound (an exact power-of-two scaling) times det
+                float jxb[3];
+                cross3(Jv, Bc, jxb);
+                const float b2 = fmaf(Bc[0], Bc[0], fmaf(Bc[1], Bc[1], Bc[2] * Bc[2]));
+                const float nc2 = fmaf(jxb[0], jxb[0], fmaf(jxb[1], jxb[1], jxb[2] * jxb[2]));
+                const float nj2 = fmaf(Jv[0], Jv[0], fmaf(Jv[1], Jv[1], Jv[2] * Jv[2]));
+                const float den2 = nj2 * b2;
                 const int bound = (ii == 0) + (jj == 0) + (kk == 0) + (ii == g.nx - 1) + (jj == g.ny - 1) +
                                   (kk == g.nz - 1);
-                e_sum += ldexp(b2 * detv, -bound);   // |B|^2 det / 2^bound, exact scaling
-                jmax = nb_max(jmax, nj2);
-                const float fnc2 = (float)nc2, fb2 = (float)b2;
-                r1 = nb_max(r1, (double)(fnc2 * __builtin_amdgcn_rcpf(fb2 * fb2)));
-                r2 = nb_max(r2, den2 > 0 ? (double)(fnc2 * __builtin_amdgcn_rcpf((float)den2)) : 0.0);
-                r3 = nb_max(r3, nc2);
-                r4 = nb_max(r4, den2);
+                const float wb = __builtin_bit_cast(float, (127 - bound) << 23);   // 2^-bound
+                e_sum = fma((double)(b2 * wb), detv, e_sum);
+                jmax2 = nb_maxf(jmax2, nj2);
+                r1q = nb_maxf(r1q, nc2 * __builtin_amdgcn_rcpf(b2 * b2));
+                r2q = nb_maxf(r2q, den2 > 0.f ? nc2 * __builtin_amdgcn_rcpf(den2) : 0.f);
+                r3q = nb_maxf(r3q, nc2);
+                r4q = nb_maxf(r4q, den2);
             }
             if constexpr (!FINAL) {
-                if constexpr (stage >= 0) {
-                    bK[stage].st3(kv, 12 * (int)a);
-                }
+#pragma unroll
+                for (int l = 0; l < 3; ++l) kst[l] = kv[l];
             } else {
                 // x_new = x_5 + sum_m (c_m - b5_m) k_m (k_5 included), into
                 // (hi, lo) by a two-sum; error |sum_m (c_m - c*_m) k_m| in FP64
-                const BufF bXhd = buf_of(A.Xh[cur ^ 1], P3), bXld = buf_of(A.Xl[cur ^ 1], P3);
                 const int va = 12 * (int)a;
                 Pt3 xc;
                 L.ldx(pj, t, xc);
-                float hn[3], ln[3];
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
                     float hi = xc.h[l], lo;
@@ -577,9 +609,13 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     hn[l] = hi;
                     ln[l] = lo;
                 }
-                bXhd.st3(hn, va);
-                bXld.st3(ln, va);
             }
+        }
+        if constexpr (!FINAL) {
+            if constexpr (stage >= 0) bK[stage].st3(kst, st_off);
+        } else {
+            bXhd.st3(hn, st_off);
+            bXld.st3(ln, st_off);
         }
 
         // one barrier per plane
@@ -599,14 +635,26 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 fe[0][l] = fe[1][l]; fe[1][l] = fe[2][l];
             }
         }
+    };
+    const int p_first = Ks - 3 - kPF, p_last = Ke + 1;
+    if constexpr (kPF == 1) {
+        for (int p = p_first; p <= p_last; ++p) plane_step(p, pf0);
+    } else {
+        int p = p_first - ((p_first + 1) & 1);   // plane p+1 even: slot 0 first
+        for (; p + 1 <= p_last; p += 2) {
+            plane_step(p, pf0);
+            plane_step(p + 1, pf1);
+        }
+        if (p <= p_last) plane_step(p, pf0);
     }
 
+    double jmax = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     if (DIAG && A.diag) {   // the maxima were kept squared
-        jmax = sqrt(jmax);
-        r1 = sqrt(r1);
-        r2 = sqrt(r2);
-        r3 = sqrt(r3);
-        r4 = sqrt(r4);
+        jmax = sqrt((double)jmax2);
+        r1 = sqrt((double)r1q);
+        r2 = sqrt((double)r2q);
+        r3 = sqrt((double)r3q);
+        r4 = sqrt((double)r4q);
     }
     // ---- block partials (fixed order: deterministic)
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
