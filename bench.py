@@ -53,7 +53,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_braid129x129x385_hbm_traf
 
 
 def measured_traffic():
-    """{prec: (tile tuple, bytes per launch)} from the committed counter summary."""
+    """{prec: (per-stage tiles, bytes per launch)} from the committed counter
+    summary; the tiles are (tx, ty, cz, workgroups) of stages 0..5."""
     try:
         lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
@@ -61,16 +62,31 @@ def measured_traffic():
     tiles, tr = {}, {}
     for l in lines:
         f = l.split()
-        if len(f) == 5 and f[0].startswith("tile_"):
-            tiles[f[0][5:]] = tuple(int(x) for x in f[1:])
+        if len(f) == 5 and f[0].startswith("tile_") and "_stage" in f[0]:
+            prec, st = f[0][5:].split("_stage")
+            tiles.setdefault(prec, {})[int(st)] = tuple(int(x) for x in f[1:])
         elif len(f) == 2 and f[0].startswith("traffic_bytes_per_launch_"):
             tr[f[0].rsplit("_", 1)[1]] = float(f[1])
-    return {k: (tiles.get(k), v) for k, v in tr.items()}
+    return {k: ([tiles.get(k, {}).get(st) for st in range(6)], v) for k, v in tr.items()}
 
 
-def traffic_for(traffic, prec, tile):
+def traffic_for(traffic, prec, stage_tiles):
+    """The committed traffic figure if it was measured with these six stage tiles."""
     t = traffic.get(prec)
-    return t[1] if t and t[0] == tuple(tile) else None
+    return t[1] if t and t[0] == [tuple(x) for x in stage_tiles] else None
+
+
+def threads32(rect):
+    """Workgroup size of an F32K stage launch for a rect of (tx+4)(ty+4) points."""
+    return 256 if rect <= 256 else 384 if rect <= 384 else 512 if rect <= 512 else 768 if rect <= 768 else 1024
+
+
+def kernel_name(stage_tiles, press=False):
+    parts = []
+    for st, (tx, ty, cz, nb) in enumerate(stage_tiles):
+        parts.append(f"stage {st}: k_stage32<{threads32((tx + 4) * (ty + 4))},{st},{str(press).lower()}> "
+                     f"{tx}x{ty} tiles, {cz}-plane chunks, {nb} workgroups")
+    return "; ".join(parts)
 
 
 def measure(g, p, B0, xb, precision, steps, warmup, dt, autotune=False):
@@ -95,6 +111,7 @@ def measure(g, p, B0, xb, precision, steps, warmup, dt, autotune=False):
     st2 = ctx.run(max_attempts=steps, n_cycle=10 ** 9, batch=steps)
     t3 = time.perf_counter()
     tile = ctx.tile()
+    stage_tiles = [ctx.stage_tile(st) for st in range(6)]
     ctx.close()
     per_stage = [ms[n] / max(1, cnt[n]) for n in range(6)]
     return {
@@ -106,6 +123,7 @@ def measure(g, p, B0, xb, precision, steps, warmup, dt, autotune=False):
         "ms_controller": ms[6] / max(1, cnt[6]),
         "launches": int(sum(cnt[:6])),
         "tile": tuple(tile),
+        "stage_tiles": stage_tiles,
         "autotune": tuned,
     }
 
@@ -128,24 +146,42 @@ def relaxation_probe(g, p, B0, xb, seconds):
             "energy": st.energy, "time_to_relax_s": sec if st.converged else None}
 
 
-def ensemble(g, workloads, n, steps):
+def ensemble(g, workloads, n, steps, autotune=True):
+    """The 64-member ensemble: tiles from the ensemble autotuner, `steps`
+    attempts per member timed as a whole, then per-launch event timing of
+    the stage launches over further attempts for the kernel roofline."""
     ps = [workloads.ensemble_member(i) for i in range(n)]
     pts = sum(p.nx * p.ny * p.nz for p in ps)
     b = g.Batch(ps)
     for m, p in zip(b.members, ps):
         B0, xb = g.init_state(p)
         m.upload(B0, xb, 0.0, 0.01)
+    tuned = b.autotune(reps=2) if autotune else None
     b.run(max_attempts=3, batch=3, n_cycle=10 ** 9)             # warm-up
     t0 = time.perf_counter()
     st = b.run(max_attempts=steps, batch=steps, n_cycle=10 ** 9)
     sec = time.perf_counter() - t0
     att = sum(s.n_attempt for s in st) - 3 * n
+    b.timing(True)
+    b.run(max_attempts=steps, batch=steps, n_cycle=10 ** 9)
+    ms, cnt = b.timing_read()
+    b.timing(False)
     b.close()
     sizes = sorted({(p.nx, p.ny, p.nz) for p in ps})
+    per_stage = [ms[k] / max(1, cnt[k]) for k in range(6)]
+    mean_ms = sum(ms[:6]) / max(1, sum(cnt[:6]))
+    achieved = 80.0 * pts / (mean_ms * 1e-3) / 1e9
     return {"members": n, "inner_points": pts, "sizes": [list(s) for s in sizes],
             "attempts_per_member": steps, "seconds": sec,
             "value": sum(p.nx * p.ny * p.nz * 6 * (s.n_attempt - 3) for p, s in zip(ps, st)) / sec,
-            "unit": "grid-pt RK-stages/s", "attempts_total": int(att)}
+            "unit": "grid-pt RK-stages/s", "attempts_total": int(att),
+            "tiles": "ensemble autotuner (gm_batch_autotune)" if autotune else "default",
+            "autotune_stage_ms": tuned,
+            "roofline": {"kernel": "k_stage32_batch (one launch per stage over all members)",
+                         "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "unit": "GB/s",
+                         "mean_launch_ms": mean_ms, "per_stage_ms": per_stage,
+                         "controller_ms": ms[6] / max(1, cnt[6]),
+                         "algorithmic_bytes_per_launch": 80.0 * pts}}
 
 
 def time_to_relax(g, theta):
@@ -168,7 +204,8 @@ def time_to_relax(g, theta):
 def large_run(g, workloads, steps):
     """The 257 x 257 x 769 braid with seeded amplitudes and pressure on, in
     F32K: `steps` accepted adaptive steps from the initial state, enqueued
-    in batches of 100 attempts with the device controller."""
+    in batches of 100 attempts with the device controller; then 20 attempts
+    with per-launch event timing for the kernel roofline."""
     p = workloads.large()
     n_pts = p.nx * p.ny * p.nz
     t0 = time.perf_counter()
@@ -182,19 +219,108 @@ def large_run(g, workloads, steps):
     t2 = time.perf_counter()
     log = ctx.step_log(4096)
     first = log[0] if len(log) == st.n_attempt else None
+    ctx.timing(True)
+    ctx.run(max_attempts=20, n_cycle=10 ** 9, batch=20)
+    ms, cnt = ctx.timing_read()
+    ctx.timing(False)
+    stage_tiles = [ctx.stage_tile(k) for k in range(6)]
     ctx.close()
+    per_stage = [ms[k] / max(1, cnt[k]) for k in range(6)]
+    mean_ms = sum(ms[:6]) / max(1, sum(cnt[:6]))
+    achieved = 80.0 * n_pts / (mean_ms * 1e-3) / 1e9
     return {"workload": "braid 257x257x769, seeded amplitudes (seed 3), pressure on (beta = 0.1 mean|B0|^2)",
             "precision": "F32K", "inner_points": n_pts, "accepted": int(st.n_accept),
             "attempts": int(st.n_attempt), "seconds": t2 - t1, "setup_s": t1 - t0,
             "value": n_pts * 6 * st.n_attempt / (t2 - t1), "unit": "grid-pt RK-stages/s",
             "t": st.t, "dt": st.dt,
             "energy_first": first.energy if first else None, "energy": st.energy,
-            "JxBMax_JBMax_first": first.ratio if first else None, "JxBMax_JBMax": st.ratio}
+            "JxBMax_JBMax_first": first.ratio if first else None, "JxBMax_JBMax": st.ratio,
+            "roofline": {"kernel": kernel_name(stage_tiles, True), "achieved": achieved,
+                         "frac": achieved / HBM_PEAK_GBS, "unit": "GB/s", "mean_launch_ms": mean_ms,
+                         "per_stage_ms": per_stage, "controller_ms": ms[6] / max(1, cnt[6]),
+                         "algorithmic_bytes_per_launch": 80.0 * n_pts}}
 
 
-def cpu_baseline(p, B0, xb, budget_s=20.0):
-    import oracle as o
-    threads = min(16, os.cpu_count() or 1)
+CLI_PARAMS = """&comp
+    nx = {n}, ny = {n}, nz = {nz}
+/
+&start
+    Lx = 8, Ly = 8, Lz = 48, Ox = -4, Oy = -4, Oz = -24
+    bInit = "blobs", ampl = 1, nBlobs = 6
+    blobXc = 1, -1, 1, -1, 1, -1
+    blobYc = 6*0
+    blobZc = -20, -12, -4, 4, 12, 20
+    blobScale = 6*1.4142135623730951, blobZl = 6*2
+    blobTwist = 0.7071067811865476, -0.7071067811865476, 0.7071067811865476, -0.7071067811865476,
+                0.7071067811865476, -0.7071067811865476
+/
+&run
+    nCycle = {ncycle}, maxError = 1e-5, dt0 = 0.02, dtMin = 1e-9
+    epsilonProf = f, pressure = f, zUpdate = t, jMethod = "Stokes"
+/
+&io
+    nTs = {nts}, dtDump = 1e9, dtSave = 1e9
+    redB2 = t, redJMax = t, redJxB_B2Max = t
+/
+"""
+
+
+def cli_run(n_cycle=400, n_ts=50, prec="32"):
+    """The glemur executable in a fresh run directory on the 129 x 129 x 385
+    braid: its own report of the time-stepping throughput (device batches
+    between time-series rows, the final save's download and write excluded)."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "glemur_amd", "glemur")
+    d = tempfile.mkdtemp(prefix="glemur_cli_")
+    try:
+        with open(os.path.join(d, "params.in"), "w") as f:
+            f.write(CLI_PARAMS.format(n=129, nz=385, ncycle=n_cycle, nts=n_ts))
+        os.makedirs(os.path.join(d, "data"))
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "-p", prec], cwd=d, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        m = re.search(r"time stepping: (\d+) attempted steps \((\d+) accepted\) in ([0-9.]+) s "
+                      r"\(dumps and saves: ([0-9.]+) s more\), ([0-9.e+-]+) grid-pt", r.stdout)
+        if r.returncode != 0 or not m:
+            return {"error": (r.stdout + r.stderr)[-400:]}
+        return {"workload": f"braid 129x129x385 through the glemur executable (-p {prec}), nCycle = {n_cycle}, "
+                            f"nTs = {n_ts}", "attempts": int(m.group(1)), "accepted": int(m.group(2)),
+                "stepping_s": float(m.group(3)), "save_s": float(m.group(4)), "value": float(m.group(5)),
+                "unit": "grid-pt RK-stages/s", "process_wall_s": wall}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def host_facts():
+    """CPU model, logical CPUs, physical cores and the CPUs this process may
+    run on (the box's share), from lscpu and the scheduler."""
+    import subprocess
+    facts = {"nproc": os.cpu_count(), "cpus_allowed": len(os.sched_getaffinity(0)),
+             "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        facts["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "1") or 1)
+        cores = int(kv.get("Core(s) per socket", "0") or 0)
+        facts["physical_cores"] = sockets * cores if cores else None
+        facts["threads_per_core"] = int(kv.get("Thread(s) per core", "1") or 1)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        facts["model"] = None
+        facts["physical_cores"] = None
+    return facts
+
+
+def cpu_rate(o, p, B0, xb, threads, budget_s, max_steps):
+    """Attempted FP64 oracle steps of the braid from its initial state at
+    `threads` OpenMP threads until budget_s or max_steps; grid-pt RK-stages/s."""
     o.set_threads(threads)
     sim = o.Sim(o.from_product(p), B0, xb)
     n_pts = p.nx * p.ny * p.nz
@@ -203,16 +329,33 @@ def cpu_baseline(p, B0, xb, budget_s=20.0):
     while True:
         sim.step()
         n += 1
-        if time.perf_counter() - t0 > budget_s or n >= 50:
+        if time.perf_counter() - t0 > budget_s or n >= max_steps:
             break
-    dt = time.perf_counter() - t0
+    sec = time.perf_counter() - t0
+    return n_pts * 6 * n / sec, n, sec
+
+
+def cpu_baseline(p, B0, xb):
+    """The FP64 oracle (gcc -O3 -fopenmp -ffp-contract=off, a C restatement
+    of the reference step) on this host: one thread and every CPU of the
+    process's share (at most the physical cores)."""
+    import oracle as o
+    facts = host_facts()
+    phys = facts.get("physical_cores") or facts["cpus_allowed"]
+    allc = max(1, min(facts["cpus_allowed"], phys))
+    one, n1, s1 = cpu_rate(o, p, B0, xb, 1, 12.0, 4)
+    full, nf, sf = cpu_rate(o, p, B0, xb, allc, 15.0, 50)
     return {
-        "value": n_pts * 6 * n / dt,
+        "value": full,
         "unit": "grid-pt RK-stages/s",
-        "cores": threads,
+        "cores": allc,
         "kind": "port",
-        "sample": f"{n} attempted FP64 steps of the 129x129x385 braid from its initial state "
-                  f"(oracle, gcc -O2 -fopenmp, {threads} OpenMP threads, {dt:.1f} s)",
+        "sample": f"{nf} attempted FP64 steps of the 129x129x385 braid from its initial state "
+                  f"(oracle, gcc -O3 -fopenmp -ffp-contract=off, {allc} OpenMP threads, {sf:.1f} s); "
+                  f"1 thread: {n1} steps in {s1:.1f} s",
+        "value_1thread": one,
+        "host": facts,
+        "flags": "gcc -O3 -fPIC -fopenmp -ffp-contract=off -fno-fast-math -std=c11",
     }
 
 
@@ -229,10 +372,13 @@ def main():
                     help="time-to-relax threshold for the cold/warm-start comparison (0: skip)")
     ap.add_argument("--large-steps", type=int, default=2000,
                     help="accepted steps of the 257x257x769 pressure-on braid (0: skip)")
+    ap.add_argument("--no-cli", dest="cli", action="store_false",
+                    help="skip the run of the glemur executable on the same braid")
     ap.add_argument("--autotune", action="store_true",
                     help="pick each stage kernel's tile by timing candidates before measuring")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="run a 1 GiB device copy first (reference kernel for HBM byte counters)")
+                    help="run 1 GiB device copy, read and write probes first (reference kernels for "
+                         "the HBM byte counters and the achievable bandwidth)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is design only "
@@ -244,8 +390,10 @@ def main():
 
     copy_gbs = None
     if args.copy_calibration:
-        ms = g.copy_probe(1 << 30, 5)
-        copy_gbs = 2 * (1 << 30) / (ms * 1e-3) / 1e9
+        # yardsticks for the byte counters and the achievable bandwidth: a
+        # 1 GiB copy, read and write, eight 16-byte words in flight per lane
+        copy_gbs = {name: mult * (1 << 30) / (g.hbm_probe(1 << 30, mode, 5) * 1e-3) / 1e9
+                    for mode, name, mult in ((0, "copy", 2), (1, "read", 1), (2, "write", 1))}
 
     p = workloads.medium()
     B0, xb = g.init_state(p)
@@ -261,7 +409,7 @@ def main():
     achieved = bytes_launch / (f32["ms_stage"] * 1e-3) / 1e9
     stage_bytes = [(48 + 12 * n) * n_pts for n in range(5)] + [120 * n_pts]
     traffic = measured_traffic()
-    traffic_f32 = traffic_for(traffic, "f32k", f32["tile"])
+    traffic_f32 = traffic_for(traffic, "f32k", f32["stage_tiles"])
 
     out = {
         "metric": "grid-pt RK-stages/s",
@@ -278,11 +426,12 @@ def main():
         "data": "synthetic (closed-form braid field, no checkpoint)",
         "config": {"workload": "braid 129x129x385 (6 blobs, line-tied, friction 1, tol 1e-5)",
                    "grid": [p.nx, p.ny, p.nz], "inner_points": n_pts,
-                   "tile_tx_ty_cz_workgroups": list(f32["tile"]), "parallelism": "single GPU"},
+                   "stage_tiles_tx_ty_cz_workgroups": [list(x) for x in f32["stage_tiles"]],
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_f32,
-                     "kernel": "k_stage32<NT,n,false> (fused Cash-Karp stage n, F32K; NT = 1024 for stages 0-4, "
-                               "768 for stage 5 at the default tiles), mean of stages 0-5",
+                     "kernel": "fused Cash-Karp stage kernels, F32K, mean of stages 0-5 (" +
+                               kernel_name(f32["stage_tiles"]) + ")",
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "algorithmic_bytes_per_point": bpp,
                      "mean_launch_ms": f32["ms_stage"],
@@ -296,7 +445,7 @@ def main():
         out["autotune"] = {"stage_tiles_tx_ty_cz_workgroups": f32["autotune"]["tiles"],
                            "stage_ms": f32["autotune"]["ms"]}
     if copy_gbs is not None:
-        out["copy_probe_GBs"] = copy_gbs
+        out["hbm_probe_GBs"] = copy_gbs
     if not args.no_f64:
         f64 = measure(g, p, B0, xb, g.PREC_F64, max(50, args.steps // 3), args.warmup, dt0)
         b64 = g.algorithmic_bytes(g.PREC_F64) * n_pts
@@ -305,13 +454,15 @@ def main():
                       "ms_per_step": 1e3 * f64["seconds"] / f64["attempts"],
                       "achieved_GBs": a64, "frac": a64 / HBM_PEAK_GBS, "mean_launch_ms": f64["ms_stage"],
                       "per_stage_ms": f64["ms_per_stage"],
-                      "traffic": traffic_for(traffic, "f64", f64["tile"])}
+                      "traffic": traffic_for(traffic, "f64", f64["stage_tiles"])}
     if args.relax_seconds > 0:
         out["relaxation"] = relaxation_probe(g, p, B0, xb, args.relax_seconds)
     if args.ensemble > 0:
         out["ensemble"] = ensemble(g, workloads, args.ensemble, 20)
     if args.large_steps > 0:
         out["large"] = large_run(g, workloads, args.large_steps)
+    if args.cli:
+        out["cli"] = cli_run()
     if args.ttr > 0:
         out["time_to_relax"] = time_to_relax(g, args.ttr)
     if not args.no_cpu:

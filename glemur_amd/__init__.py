@@ -96,14 +96,14 @@ class StepInfo(C.Structure):
 
 class RunOpts(C.Structure):
     _fields_ = [("max_attempts", C.c_int64), ("n_cycle", C.c_int), ("stop_ratio", C.c_double),
-                ("batch", C.c_int), ("use_graph", C.c_int)]
+                ("batch", C.c_int), ("use_graph", C.c_int), ("dt_dump", C.c_double), ("dt_save", C.c_double)]
 
 
 class RunStats(C.Structure):
     _fields_ = [("n_attempt", C.c_int64), ("n_accept", C.c_int64), ("t", C.c_double), ("dt", C.c_double),
                 ("ratio", C.c_double), ("ratio_pt", C.c_double), ("energy", C.c_double),
                 ("converged", C.c_int), ("stopped", C.c_int),
-                ("nan_seen", C.c_int), ("seconds", C.c_double)]
+                ("nan_seen", C.c_int), ("seconds", C.c_double), ("event", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -162,6 +162,7 @@ _SIGS = {
     "gm_algorithmic_bytes_per_point_stage": (C.c_double, [C.c_int]),
     "gm_set_tile": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_int]),
     "gm_copy_probe": (C.c_int, [C.c_int64, C.c_int, _D]),
+    "gm_hbm_probe": (C.c_int, [C.c_int64, C.c_int, C.c_int, _D]),
     "gm_get_tile": (C.c_int, [_CTX, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "gm_write_b0": (C.c_int, [C.c_char_p, _P(Params), _D, C.c_int]),
     "gm_dump_state": (C.c_int, [C.c_char_p, _P(Params), C.c_double, C.c_double, C.c_int,
@@ -172,6 +173,10 @@ _SIGS = {
     "gm_write_device_info": (C.c_int, [C.c_char_p, C.c_int, C.c_int]),
     "gm_get_stage_tile": (C.c_int, [_CTX, C.c_int, _P(C.c_int), _P(C.c_int), _P(C.c_int), _P(C.c_int)]),
     "gm_autotune": (C.c_int, [_CTX, C.c_int, _D]),
+    "gm_set_stage_tile": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "gm_batch_autotune": (C.c_int, [_CTX, C.c_int, _D]),
+    "gm_batch_timing_enable": (C.c_int, [_CTX, C.c_int]),
+    "gm_batch_timing_read": (C.c_int, [_CTX, _D, _P(C.c_int64)]),
     "gm_batch_create": (C.c_int, [_P(Params), C.c_int, C.c_int, _P(_CTX)]),
     "gm_batch_destroy": (C.c_int, [_CTX]),
     "gm_batch_context": (C.c_int, [_CTX, C.c_int, _P(_CTX)]),
@@ -254,6 +259,14 @@ def copy_probe(nbytes, reps=10):
     """Mean ms of a plain device copy of nbytes (reads nbytes, writes nbytes)."""
     ms = C.c_double()
     _check(_lib.gm_copy_probe(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
+
+
+def hbm_probe(nbytes, mode, reps=10):
+    """Mean ms of a device copy (mode 0: reads and writes nbytes), read
+    (1: nbytes) or write (2: nbytes) with eight 16-byte words in flight per lane."""
+    ms = C.c_double()
+    _check(_lib.gm_hbm_probe(int(nbytes), int(mode), int(reps), C.byref(ms)))
     return ms.value
 
 
@@ -391,6 +404,10 @@ class Context:
         _check(_lib.gm_get_stage_tile(self._h, stage, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
+    def set_stage_tile(self, stage, tx, ty, cz):
+        """Tile of one stage kernel (-1: fields pass): tx x ty columns, cz planes."""
+        _check(_lib.gm_set_stage_tile(self._h, stage, tx, ty, cz))
+
     def autotune(self, reps=3):
         """Pick each stage kernel's tile by timing candidates; returns the
         winners' mean launch times (ms) for stages 0..5."""
@@ -431,6 +448,23 @@ class Batch:
         st = (RunStats * len(self.members))()
         _check(_lib.gm_batch_run(self._h, C.byref(o), st))
         return [st[i] for i in range(len(self.members))]
+
+    def timing(self, on=True):
+        _check(_lib.gm_batch_timing_enable(self._h, int(on)))
+
+    def timing_read(self):
+        """(ms, count) summed per kind: stage launches 0..5, then the controller."""
+        ms = (C.c_double * TIMING_KINDS)()
+        cnt = (C.c_int64 * TIMING_KINDS)()
+        _check(_lib.gm_batch_timing_read(self._h, ms, cnt))
+        return list(ms), list(cnt)
+
+    def autotune(self, reps=3):
+        """Pick every stage's tile policy for the whole ensemble by timing
+        candidates; returns the winners' mean launch times (ms), stages 0..5."""
+        ms = (C.c_double * 6)()
+        _check(_lib.gm_batch_autotune(self._h, int(reps), ms))
+        return list(ms)
 
     def close(self):
         if self._h:

@@ -10,6 +10,62 @@
 #include "gm_fused.hpp"
 #include "gm_internal.hpp"
 
+// HIP-event timing of kernel launches by kind (0..5 the stage kernels, 6
+// the step controller): a pair of events around each launch, folded into
+// per-kind totals after a stream synchronisation.
+struct EvTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    int used = 0;
+    double ms[GM_TIMING_KINDS] = {};
+    long long count[GM_TIMING_KINDS] = {};
+    void begin(hipStream_t s, int k)
+    {
+        if (!on) return;
+        if (used + 2 > (int)ev.size()) {
+            for (int i = 0; i < 64; ++i) {
+                hipEvent_t e;
+                (void)hipEventCreate(&e);
+                ev.push_back(e);
+                kind.push_back(0);
+            }
+        }
+        kind[used] = k;
+        (void)hipEventRecord(ev[used], s);
+        used++;
+    }
+    void end(hipStream_t s)
+    {
+        if (!on) return;
+        (void)hipEventRecord(ev[used], s);
+        used++;
+    }
+    // after a synchronisation of the stream the events were recorded on
+    void collect()
+    {
+        for (int i = 0; i + 1 < used; i += 2) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, ev[i], ev[i + 1]) == hipSuccess) {
+                ms[kind[i]] += t;
+                count[kind[i]] += 1;
+            }
+        }
+        used = 0;
+    }
+    void reset()
+    {
+        for (int i = 0; i < GM_TIMING_KINDS; ++i) { ms[i] = 0; count[i] = 0; }
+    }
+    void destroy()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        kind.clear();
+        used = 0;
+    }
+};
+
 struct gm_ctx {
     gm_params p{};
     int prec = GM_PREC_F64;
@@ -59,13 +115,7 @@ struct gm_ctx {
     int TX[7] = {}, TY[7] = {}, tiles_x[7] = {}, tiles_y[7] = {};
     int CZ[7] = {}, nchunks[7] = {}, nblocks[7] = {};
     int ncu = 256;
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> ev_kind;
-    int ev_used = 0;
-    double tms[GM_TIMING_KINDS] = {};        // per stage 0..5, controller
-    long long tcount[GM_TIMING_KINDS] = {};
+    EvTimer tm;                   // kernel timing (gm_timing_*)
     // graph of one batch
     hipGraphExec_t gexec = nullptr;
     int gbatch = 0;

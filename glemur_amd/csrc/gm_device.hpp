@@ -43,8 +43,17 @@ struct Ctl {
     long long max_attempt, n_cycle;      // run limits (n_cycle counts accepted steps)
     double stop_ratio;
     int diag_valid;
-    int pad;
+    int event;              // the last accepted step is due a dump or a save
+    double ev_dump, ev_save;  // dump / save intervals for `event` (<= 0: off)
 };
+
+// src/main.cu:362-376: after an accepted step a dump (save) is due when
+// (t - dt, t] holds a multiple of the interval, dt being the already
+// adapted step of the next attempt
+__host__ __device__ inline bool interval_crossed(double t, double dt, double every)
+{
+    return every > 0 && ceil((t - dt) / every) != ceil(t / every);
+}
 
 struct StepRecord {
     double t, dt_used, maxDelta, ratio, ratio_pt, energy, jxb_b2, jmax;
@@ -113,7 +122,8 @@ void launch_aos_to_soa_f32(hipStream_t s, long long P, const double *src, float 
 // slopes)
 template <class R>
 void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, double *dst, bool triples = false);
-void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long bytes);
+// mode 0: copy src -> dst, 1: read src, 2: write dst; `bytes` per array
+void launch_hbm_probe(hipStream_t s, int mode, const void *src, void *dst, long long bytes);
 void launch_fill_f64(hipStream_t s, double *p, long long n, double v);
 void launch_fill_f32(hipStream_t s, float *p, long long n, float v);
 

@@ -289,8 +289,9 @@ extern "C" int gm_read_state(const char *dir, const gm_params *p, double *xb, do
     return GM_OK;
 }
 
-// writeTs, src/io.cu:542-678.  jxb_jb_col adds the JxB_JBMax column after
-// the reference's columns (python/glemur/ts.py takes names from the header).
+// writeTs, src/io.cu:542-678.  jxb_jb_col adds two columns after the
+// reference's: JxB_JBMax and JxBMax_JBMax (python/glemur/ts.py takes the
+// names from the header).
 extern "C" int gm_write_ts(const char *dir, const gm_params *p, const gm_reduction *red, double t, double dt,
                            int it, double maxDelta, int echo, int jxb_jb_col)
 {

@@ -658,9 +658,11 @@ __device__ __forceinline__ void controller_body(const CtlArgs &A)
     rec.t = ctl->t;
     if (A.log) A.log[ctl->n_attempt % A.log_cap] = rec;
     ctl->n_attempt += 1;
+    ctl->event = acc && (interval_crossed(ctl->t, ctl->dt, ctl->ev_dump) || interval_crossed(ctl->t, ctl->dt, ctl->ev_save));
     if (ctl->n_accept >= ctl->n_cycle) { ctl->done = 1; ctl->stop_reason = 1; }
     else if (!(ctl->dt > A.dtMin)) { ctl->done = 1; ctl->stop_reason = 2; }
     else if (ctl->n_attempt >= ctl->max_attempt) { ctl->done = 1; ctl->stop_reason = 4; }
+    else if (ctl->event) { ctl->done = 1; ctl->stop_reason = 5; }
 }
 
 __global__ void __launch_bounds__(1024) k_controller(CtlArgs A) { controller_body(A); }

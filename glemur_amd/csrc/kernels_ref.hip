@@ -759,10 +759,53 @@ __global__ void k_padded_to_inner_aos(Geo g, const R *src, double *dst, bool tri
 }
 // Plain device copy: the reference kernel for HBM byte counters and the
 // achievable-bandwidth probe.
-__global__ void __launch_bounds__(256) k_copy_probe(const float4 *src, float4 *dst, long long n)
+// HBM yardsticks.  Each workgroup owns kProbeU * 256 consecutive 16-byte
+// words; a lane issues its kProbeU loads (one per 4 KB slice, coalesced
+// across the wave) before any store, so kProbeU * 16 bytes per lane are in
+// flight, and the grid holds one workgroup per slice set (no grid-stride
+// loop whose trips would serialise on the load latency).
+constexpr int kProbeU = 8;
+__global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                    long long n)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const long long base = (long long)blockIdx.x * (kProbeU * 256) + threadIdx.x;
+    float4 v[kProbeU];
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u) {
+        const long long i = base + u * 256;
+        v[u] = i < n ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u) {
+        const long long i = base + u * 256;
+        if (i < n) dst[i] = v[u];
+    }
+}
+// read only: the loaded words are summed; the sum is stored only if it is
+// NaN (never, for the zero-filled source), which keeps the loads live
+__global__ void __launch_bounds__(256) k_read_probe(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                    long long n)
+{
+    const long long base = (long long)blockIdx.x * (kProbeU * 256) + threadIdx.x;
+    float4 v[kProbeU];
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u) {
+        const long long i = base + u * 256;
+        v[u] = i < n ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    float a = 0.f;
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u) a += (v[u].x + v[u].y) + (v[u].z + v[u].w);
+    if (a != a) dst[base] = make_float4(a, a, a, a);
+}
+__global__ void __launch_bounds__(256) k_write_probe(float4 *__restrict__ dst, long long n)
+{
+    const long long base = (long long)blockIdx.x * (kProbeU * 256) + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < kProbeU; ++u) {
+        const long long i = base + u * 256;
+        if (i < n) dst[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 __global__ void k_fill_f64(double *p, long long n, double v)
 {
@@ -890,10 +933,16 @@ void launch_padded_to_inner_aos(hipStream_t s, const Geo &g, const R *src, doubl
 }
 template void launch_padded_to_inner_aos<double>(hipStream_t, const Geo &, const double *, double *, bool);
 template void launch_padded_to_inner_aos<float>(hipStream_t, const Geo &, const float *, double *, bool);
-void launch_copy_probe(hipStream_t s, const void *src, void *dst, long long bytes)
+void launch_hbm_probe(hipStream_t s, int mode, const void *src, void *dst, long long bytes)
 {
-    long long n = bytes / 16;
-    hipLaunchKernelGGL(k_copy_probe, dim3(256 * 32), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n);
+    const long long n = bytes / 16;
+    const long long blocks = (n + kProbeU * 256 - 1) / (kProbeU * 256);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_read_probe, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n);
+    else if (mode == 2)
+        hipLaunchKernelGGL(k_write_probe, dim3((unsigned)blocks), dim3(256), 0, s, (float4 *)dst, n);
+    else
+        hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)src, (float4 *)dst, n);
 }
 void launch_fill_f64(hipStream_t s, double *p, long long n, double v)
 {

@@ -8,6 +8,8 @@
 //     -p 32  FP32 stages with compensated positions
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +19,13 @@
 #include <vector>
 
 #include "../../include/glemur.h"
+
+// src/main.cu:362-376: a dump (save) is due after an accepted step when
+// (t - dt, t] holds a multiple of the interval, dt being the adapted step
+static bool gm_interval_due(double t, double dt, double every)
+{
+    return every > 0 && ceil((t - dt) / every) != ceil(t / every);
+}
 
 static int die(const char *what)
 {
@@ -99,51 +108,118 @@ int main(int argc, char *argv[])
     if (gm_upload(ctx, B0.data(), xb.data(), t, dt) != GM_OK) return die("upload");
     if (p.inertia && p.fRestart && gm_upload_velocity(ctx, uu.data()) != GM_OK) return die("upload velocity");
 
-    // time stepping, src/main.cu:259-384
-    int timeIndex;
-    for (timeIndex = 0; timeIndex < p.nCycle && dt > p.dtMin; timeIndex++) {
-        gm_step_info s{};
-        int r = gm_step(ctx, &s);
-        if (r == GM_ERR_NAN) {
-            printf("error: %s\n", gm_last_error());
-            break;
+    // time stepping, src/main.cu:259-384.  The device controller runs the
+    // attempts in batches without host round trips (gm_run); a batch ends at
+    // the first accepted step that is due a dump or a save (the reference's
+    // rule, the event stop of gm_run), after nCycle accepted steps, when dt
+    // falls to dtMin, after 4096 attempts (the step log's capacity) and, when
+    // a time-series column only the separate reductions give is requested,
+    // at every nTs-th accepted step.  The other rows come from the step log:
+    // the diagnostics recorded at the start of the attempt after an accepted
+    // step are those of the state it produced.
+    const bool extra = p.redEpsilonStar || p.redErrB_1ez || p.redErrXb_XbAn || p.redConvex || p.redWedgeMin ||
+                       (p.inertia && (p.redU2 || p.redUMax));
+    const int log_cap = 4096;
+    std::vector<gm_step_record> recs(log_cap);
+    int timeIndex = 0;
+    // a due row whose diagnostics the next attempt's record will carry
+    bool pending = false;
+    int pend_it = 0;
+    double pend_t = 0, pend_dt = 0, pend_md = 0;
+    auto row_from = [&](const gm_step_record &r) {
+        gm_reduction q{};
+        q.B2 = r.energy;
+        q.JMax = r.jmax;
+        q.JxB_B2Max = r.jxb_b2;
+        q.JxB_JBMax = r.ratio_pt;
+        q.JxBMax_JBMax = r.ratio;
+        q.B2res = red.B2res;
+        gm_write_ts(".", &p, &q, pend_t, pend_dt, pend_it, pend_md, 1, 1);
+        pending = false;
+    };
+    auto row_now = [&](int it, double tt, double dtu, double md) -> int {
+        if (gm_fields(ctx) != GM_OK || gm_reductions(ctx, &red) != GM_OK) return -1;
+        gm_write_ts(".", &p, &red, tt, dtu, it, md, 1, 1);
+        return 0;
+    };
+    bool nan_stop = false;
+    int64_t attempts_done = 0;
+    const auto clock0 = std::chrono::steady_clock::now();
+    double io_secs = 0;   // dumps and saves (downloads and file writes)
+    while (timeIndex < p.nCycle && dt > p.dtMin && !nan_stop) {
+        gm_run_opts o{};
+        o.max_attempts = log_cap;
+        int to_cycle = p.nCycle - timeIndex;
+        if (extra) to_cycle = std::min(to_cycle, p.nTs - timeIndex % p.nTs);
+        o.n_cycle = to_cycle;
+        o.batch = 64;
+        o.dt_dump = p.dtDump;
+        o.dt_save = p.dtSave;
+        gm_run_stats st{};
+        const int r = gm_run(ctx, &o, &st);
+        if (r != GM_OK && r != GM_ERR_NAN) return die("step");
+        // this batch's attempts, oldest first
+        const int n = (int)(st.n_attempt - attempts_done);
+        attempts_done = st.n_attempt;
+        const int got = gm_step_log(ctx, recs.data(), std::min(n, log_cap));
+        if (got < 0) return die("step log");
+        for (int k = 0; k < got; ++k) {
+            const gm_step_record &rc = recs[k];
+            if (pending && rc.ratio >= 0) row_from(rc);
+            if (rc.nan_seen) {
+                printf("error: NaN in the error estimate (the reference's max would skip it)\n");
+                nan_stop = true;
+                break;
+            }
+            if (rc.accepted == 1) {
+                if (timeIndex % p.nTs == 0) {
+                    pending = true;
+                    pend_it = timeIndex;
+                    pend_t = rc.t;
+                    pend_dt = rc.dt_used;
+                    pend_md = rc.maxDelta;
+                }
+                timeIndex++;
+            }
         }
-        if (r != GM_OK) return die("step");
-        double maxDelta = s.maxDelta;
-        if (s.accepted) {
-            t = s.t;
-            bool fields = false;
-            if (timeIndex % p.nTs == 0) {
-                if (gm_fields(ctx) != GM_OK || gm_reductions(ctx, &red) != GM_OK) return die("reductions");
-                fields = true;
-                gm_write_ts(".", &p, &red, t, s.dt_used, timeIndex, maxDelta, 1, 1);
-            }
-            dt = s.dt;  // already adapted (src/main.cu:362-363)
-            bool dump = ceil((t - dt) / p.dtDump) != ceil(t / p.dtDump);
-            bool save = ceil((t - dt) / p.dtSave) != ceil(t / p.dtSave) || timeIndex == p.nCycle - 1;
-            if (dump || save) {
-                if (!fields && gm_fields(ctx) != GM_OK) return die("fields");
-                if (gm_download(ctx, GM_FIELD_XB, xb.data()) || gm_download(ctx, GM_FIELD_B, B.data()) ||
-                    gm_download(ctx, GM_FIELD_J, J.data()) || gm_download(ctx, GM_FIELD_DETJAC, det.data()))
-                    return die("download");
-                if ((p.dumpCellVol && gm_download(ctx, GM_FIELD_CELLVOL, cvol.data())) ||
-                    (p.dumpConvexity && gm_download(ctx, GM_FIELD_CONVEXITY, conv.data())) ||
-                    (p.dumpWedgeMin && gm_download(ctx, GM_FIELD_WEDGEMIN, wmin.data())) ||
-                    (p.inertia && gm_download(ctx, GM_FIELD_UU, uu.data())))
-                    return die("download");
-                const double *u = p.inertia ? uu.data() : nullptr;
-                if (dump)
-                    gm_dump_state(".", &p, t, dt, (int)ceil((t - dt) / p.dtDump), xb.data(), B.data(), u, J.data(),
-                                  det.data(), cvol.data(), conv.data(), wmin.data(), 64);
-                if (save)
-                    gm_dump_state(".", &p, t, dt, -1, xb.data(), B.data(), u, J.data(), det.data(), cvol.data(),
-                                  conv.data(), wmin.data(), 64);
-            }
-        } else {
-            timeIndex -= 1;
-            dt = s.dt;
+        t = st.t;
+        dt = st.dt;
+        if (nan_stop) break;
+        const bool last = timeIndex >= p.nCycle;
+        const bool dump = st.event && gm_interval_due(t, dt, p.dtDump);
+        const bool save = (st.event && gm_interval_due(t, dt, p.dtSave)) || last;
+        if (pending && (extra || dump || save || last || !(dt > p.dtMin))) {
+            if (row_now(pend_it, pend_t, pend_dt, pend_md)) return die("reductions");
+            pending = false;
+        }
+        if (dump || save) {
+            const auto io0 = std::chrono::steady_clock::now();
+            if (gm_fields(ctx) != GM_OK) return die("fields");
+            if (gm_download(ctx, GM_FIELD_XB, xb.data()) || gm_download(ctx, GM_FIELD_B, B.data()) ||
+                gm_download(ctx, GM_FIELD_J, J.data()) || gm_download(ctx, GM_FIELD_DETJAC, det.data()))
+                return die("download");
+            if ((p.dumpCellVol && gm_download(ctx, GM_FIELD_CELLVOL, cvol.data())) ||
+                (p.dumpConvexity && gm_download(ctx, GM_FIELD_CONVEXITY, conv.data())) ||
+                (p.dumpWedgeMin && gm_download(ctx, GM_FIELD_WEDGEMIN, wmin.data())) ||
+                (p.inertia && gm_download(ctx, GM_FIELD_UU, uu.data())))
+                return die("download");
+            const double *u = p.inertia ? uu.data() : nullptr;
+            if (dump)
+                gm_dump_state(".", &p, t, dt, (int)ceil((t - dt) / p.dtDump), xb.data(), B.data(), u, J.data(),
+                              det.data(), cvol.data(), conv.data(), wmin.data(), 64);
+            if (save)
+                gm_dump_state(".", &p, t, dt, -1, xb.data(), B.data(), u, J.data(), det.data(), cvol.data(),
+                              conv.data(), wmin.data(), 64);
+            io_secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - io0).count();
         }
     }
+    if (pending && row_now(pend_it, pend_t, pend_dt, pend_md)) return die("reductions");
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - clock0).count() - io_secs;
+    printf("time stepping: %lld attempted steps (%d accepted) in %.3f s (dumps and saves: %.3f s more), "
+           "%.4e grid-pt RK-stages/s\n",
+           (long long)attempts_done, timeIndex, secs, io_secs,
+           secs > 0 ? 6.0 * (double)p.nx * p.ny * p.nz * (double)attempts_done / secs : 0.0);
     if (dt < p.dtMin) printf("dt < dtMin = %e, stopping simulation\n", p.dtMin);
     gm_destroy(ctx);
     printf("glemur terminated successfully\n");

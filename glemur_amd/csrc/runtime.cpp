@@ -136,8 +136,12 @@ int best_chunks(int nz, int cols, int slots)
 }
 
 // 128-byte lines a launch loads per plane of one array for tiles of
-// (tlx x tly) rects of rw x rh points: a rect row is 4 rw contiguous bytes
-// at an arbitrary offset within a line, (4 rw + 124) / 128 lines on average
+// (tlx x tly) rects of rw x rh points: a rect row of rw points at an
+// arbitrary offset within a line touches (4 rw + 124) / 128 lines on
+// average if a point is 4 bytes.  The F32K arrays hold 12-byte (x, y, z)
+// triples, but the 4-byte count is kept on purpose as a ranking heuristic:
+// counting 12 bytes picks the same tiles at 129 x 129 x 385 and moves the
+// other workloads by under 1.5% either way
 double rect_lines(int tlx, int tly, int rw, int rh)
 {
     return (double)tlx * tly * rh * (4.0 * rw + 124.0) / 128.0;
@@ -232,39 +236,8 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
     return A;
 }
 
-void ev_begin(gm_ctx *c, int kind)
-{
-    if (!c->timing) return;
-    if (c->ev_used + 2 > (int)c->ev.size()) {
-        for (int i = 0; i < 64; ++i) {
-            hipEvent_t e;
-            (void)hipEventCreate(&e);
-            c->ev.push_back(e);
-            c->ev_kind.push_back(0);
-        }
-    }
-    c->ev_kind[c->ev_used] = kind;
-    (void)hipEventRecord(c->ev[c->ev_used], c->stream);
-    c->ev_used++;
-}
-void ev_end(gm_ctx *c)
-{
-    if (!c->timing) return;
-    (void)hipEventRecord(c->ev[c->ev_used], c->stream);
-    c->ev_used++;
-}
-// after a stream sync: fold recorded pairs into the totals
-void ev_collect(gm_ctx *c)
-{
-    for (int i = 0; i + 1 < c->ev_used; i += 2) {
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess) {
-            c->tms[c->ev_kind[i]] += ms;
-            c->tcount[c->ev_kind[i]] += 1;
-        }
-    }
-    c->ev_used = 0;
-}
+void ev_begin(gm_ctx *c, int kind) { c->tm.begin(c->stream, kind); }
+void ev_end(gm_ctx *c) { c->tm.end(c->stream); }
 
 // Enqueue one attempted step: stages 0..5 and the controller.
 void enqueue_step(gm_ctx *c, int diag, int check_done)
@@ -298,7 +271,7 @@ int sync_ctl(gm_ctx *c)
 {
     HIPCHK(hipMemcpyAsync(c->ctlh, c->ctl, sizeof(gmd::Ctl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    ev_collect(c);
+    c->tm.collect();
     c->cur = c->ctlh->cur;
     return GM_OK;
 }
@@ -454,7 +427,7 @@ extern "C" int gm_destroy(gm_ctx *c)
     dfree(c->red_out); dfree(c->diag_scratch); dfree(c->nanflag); dfree(c->extra_part); dfree(c->xns_part);
     if (c->ctlh) (void)hipHostFree(c->ctlh);
     if (c->red_host) (void)hipHostFree(c->red_host);
-    for (auto e : c->ev) (void)hipEventDestroy(e);
+    c->tm.destroy();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GM_OK;
@@ -956,9 +929,11 @@ static int step_per_kernel(gm_ctx *c)
     rec.t = h.t;
     if ((r = log())) return r;
     h.n_attempt += 1;
+    h.event = acc && (gmd::interval_crossed(h.t, h.dt, h.ev_dump) || gmd::interval_crossed(h.t, h.dt, h.ev_save));
     if (h.n_accept >= h.n_cycle) { h.done = 1; h.stop_reason = 1; }
     else if (!(h.dt > c->p.dtMin)) { h.done = 1; h.stop_reason = 2; }
     else if (h.n_attempt >= h.max_attempt) { h.done = 1; h.stop_reason = 4; }
+    else if (h.event) { h.done = 1; h.stop_reason = 5; }
     c->cur = h.cur;
     return push_ctl(c);
 }
@@ -979,6 +954,8 @@ extern "C" int gm_step(gm_ctx *c, gm_step_info *info)
     h.stop_ratio = 0;
     h.n_cycle = 1LL << 62;
     h.max_attempt = 1LL << 62;
+    h.event = 0;
+    h.ev_dump = h.ev_save = 0;
     if ((r = push_ctl(c))) return r;
     if (per_kernel_path(c)) {
         if ((r = step_per_kernel(c))) return r;
@@ -1006,12 +983,15 @@ extern "C" int gm_run(gm_ctx *c, const gm_run_opts *o, gm_run_stats *st)
     h.nan_seen = 0;
     h.diag_valid = 0;
     h.stop_ratio = o->stop_ratio;
+    h.event = 0;
+    h.ev_dump = o->dt_dump;
+    h.ev_save = o->dt_save;
     long long ncyc = o->n_cycle > 0 ? o->n_cycle : c->p.nCycle;
     h.n_cycle = h.n_accept + ncyc;
     h.max_attempt = h.n_attempt + (o->max_attempts > 0 ? o->max_attempts : (1LL << 40));
     if ((r = push_ctl(c))) return r;
     int batch = std::max(1, o->batch);
-    bool graph = o->use_graph && !c->timing && !per_kernel_path(c);
+    bool graph = o->use_graph && !c->tm.on && !per_kernel_path(c);
     auto t0 = std::chrono::steady_clock::now();
     if (graph && (!c->gexec || c->gbatch != batch)) {
         if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -1050,6 +1030,7 @@ extern "C" int gm_run(gm_ctx *c, const gm_run_opts *o, gm_run_stats *st)
         st->stopped = h.done && !h.converged;
         st->nan_seen = h.nan_seen;
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->event = h.event;
     }
     if (h.nan_seen) return fail(GM_ERR_NAN, "NaN in the error estimate");
     return GM_OK;
@@ -1079,19 +1060,19 @@ extern "C" int gm_timing_enable(gm_ctx *c, int on)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
-    ev_collect(c);
-    c->timing = on != 0;
-    for (int i = 0; i < GM_TIMING_KINDS; ++i) { c->tms[i] = 0; c->tcount[i] = 0; }
+    c->tm.collect();
+    c->tm.on = on != 0;
+    c->tm.reset();
     return GM_OK;
 }
 extern "C" int gm_timing_read(gm_ctx *c, double ms[GM_TIMING_KINDS], int64_t count[GM_TIMING_KINDS])
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     HIPCHK(hipStreamSynchronize(c->stream));
-    ev_collect(c);
+    c->tm.collect();
     for (int i = 0; i < GM_TIMING_KINDS; ++i) {
-        if (ms) ms[i] = c->tms[i];
-        if (count) count[i] = c->tcount[i];
+        if (ms) ms[i] = c->tm.ms[i];
+        if (count) count[i] = c->tm.count[i];
     }
     return GM_OK;
 }
@@ -1124,22 +1105,28 @@ extern "C" int gm_get_tile(gm_ctx *c, int *tx, int *ty, int *cz, int *nblocks)
     return GM_OK;
 }
 
-// Device-to-device copy of `bytes` with a plain kernel, `reps` times; the
-// mean kernel time in ms.  No reference counterpart: the yardstick for HBM
-// byte counters and achievable bandwidth.
-extern "C" int gm_copy_probe(int64_t bytes, int reps, double *ms_mean)
+// HBM yardsticks (no reference counterpart): a device copy of `bytes`
+// (mode 0), a read of `bytes` (1) or a write of `bytes` (2), `reps` times;
+// the mean kernel time in ms.  They calibrate the HBM byte counters and
+// give the achievable bandwidth the stage kernels are compared with.
+extern "C" int gm_hbm_probe(int64_t bytes, int mode, int reps, double *ms_mean)
 {
-    if (bytes < 16 || reps < 1) return fail(GM_ERR_ARG, "gm_copy_probe: bad size");
+    if (bytes < 16 || reps < 1 || mode < 0 || mode > 2) return fail(GM_ERR_ARG, "gm_hbm_probe: bad size or mode");
+    if (bytes / 16 / 2048 >= (1LL << 31)) return fail(GM_ERR_ARG, "gm_hbm_probe: size too large");
     void *a = nullptr, *b = nullptr;
     HIPCHK(hipMalloc(&a, (size_t)bytes));
-    HIPCHK(hipMalloc(&b, (size_t)bytes));
+    if (hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(GM_ERR_HIP, "gm_hbm_probe: out of device memory");
+    }
     (void)hipMemset(a, 0, (size_t)bytes);
+    (void)hipMemset(b, 0, (size_t)bytes);
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    gmd::launch_copy_probe(nullptr, a, b, bytes);
+    gmd::launch_hbm_probe(nullptr, mode, a, b, bytes);
     (void)hipEventRecord(e0, nullptr);
-    for (int i = 0; i < reps; ++i) gmd::launch_copy_probe(nullptr, a, b, bytes);
+    for (int i = 0; i < reps; ++i) gmd::launch_hbm_probe(nullptr, mode, a, b, bytes);
     (void)hipEventRecord(e1, nullptr);
     hipError_t e = hipEventSynchronize(e1);
     float ms = 0;
@@ -1151,6 +1138,10 @@ extern "C" int gm_copy_probe(int64_t bytes, int reps, double *ms_mean)
     if (e != hipSuccess) return fail(GM_ERR_HIP, hipGetErrorString(e));
     if (ms_mean) *ms_mean = ms / reps;
     return GM_OK;
+}
+extern "C" int gm_copy_probe(int64_t bytes, int reps, double *ms_mean)
+{
+    return gm_hbm_probe(bytes, 0, reps, ms_mean);
 }
 
 // writeCudaInfo (src/io.cu:10-82): device properties, same key/value layout,
@@ -1225,7 +1216,7 @@ extern "C" int gm_autotune(gm_ctx *c, int reps, double ms_best[6])
 {
     if (!c) return fail(GM_ERR_ARG, "null context");
     if (int r = check_fused(c)) return r;
-    if (c->timing) return fail(GM_ERR_ARG, "gm_autotune: disable kernel timing first");
+    if (c->tm.on) return fail(GM_ERR_ARG, "gm_autotune: disable kernel timing first");
     reps = std::max(1, reps);
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     const int f32k = c->prec == GM_PREC_F32K;
@@ -1287,6 +1278,18 @@ extern "C" int gm_autotune(gm_ctx *c, int reps, double ms_best[6])
     return ensure_partials(c);
 }
 
+extern "C" int gm_set_stage_tile(gm_ctx *c, int stage, int tx, int ty, int cz)
+{
+    if (!c || stage < -1 || stage > 5 || tx <= 0 || ty <= 0 || cz <= 0)
+        return fail(GM_ERR_ARG, "gm_set_stage_tile: bad arguments");
+    const int k = stage < 0 ? 6 : stage;
+    if ((std::min(tx, c->p.nx) + 4) * (std::min(ty, c->p.ny) + 4) > stage_max_rect(c, k))
+        return fail(GM_ERR_ARG, "gm_set_stage_tile: rect (tx+4)(ty+4) exceeds the stage kernel's workgroup");
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    set_tile_k(c, k, tx, ty, cz);
+    return ensure_partials(c);
+}
+
 extern "C" int gm_get_stage_tile(gm_ctx *c, int stage, int *tx, int *ty, int *cz, int *nblocks)
 {
     if (!c || stage < -1 || stage > 5) return fail(GM_ERR_ARG, "gm_get_stage_tile: bad arguments");
@@ -1316,6 +1319,7 @@ struct gm_batch {
     std::vector<gmd::Ctl *> ctlh;   // pinned host mirrors (the members' own)
     hipGraphExec_t gexec = nullptr;
     int gbatch = 0;
+    EvTimer tm;                     // kernel timing (gm_batch_timing_*)
 };
 
 namespace {
@@ -1386,9 +1390,14 @@ int batch_prepare(gm_batch *b)
 void batch_enqueue_step(gm_batch *b)
 {
     const int n = (int)b->inst.size();
-    for (int s = 0; s < 6; ++s)
+    for (int s = 0; s < 6; ++s) {
+        b->tm.begin(b->stream, s);
         gmd::launch_fused32_batch(b->stream, s, b->press, b->nt[s], b->args[s], b->start[s], n, b->total[s]);
+        b->tm.end(b->stream);
+    }
+    b->tm.begin(b->stream, 6);
     gmd::launch_controller_batch(b->stream, b->cargs, n);
+    b->tm.end(b->stream);
 }
 
 }  // namespace
@@ -1397,10 +1406,10 @@ void batch_enqueue_step(gm_batch *b)
 // as long as the longest march.  Per stage kernel the members share one
 // chunk length L (a member shorter than L takes one chunk): the fewest
 // ceil(workgroups / resident slots) x (L + 5) plane steps.
-static void batch_chunks(gm_batch *b)
+static void batch_chunks_k(gm_batch *b, int k)
 {
     gm_ctx *c0 = b->inst[0];
-    for (int k = 0; k < 7; ++k) {
+    {
         const long long slots = (long long)c0->ncu * gmd::blocks_per_cu32(batch_nt(b, k), k == 6 ? -1 : k, b->press);
         int maxnz = 1;
         for (gm_ctx *c : b->inst) maxnz = std::max(maxnz, c->p.nz);
@@ -1415,6 +1424,10 @@ static void batch_chunks(gm_batch *b)
         }
         for (gm_ctx *c : b->inst) set_tile_k(c, k, c->TX[k], c->TY[k], bestL);
     }
+}
+static void batch_chunks(gm_batch *b)
+{
+    for (int k = 0; k < 7; ++k) batch_chunks_k(b, k);
 }
 
 extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm_batch **out)
@@ -1455,6 +1468,7 @@ extern "C" int gm_batch_destroy(gm_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     batch_free_dev(b);
+    b->tm.destroy();
     for (gm_ctx *c : b->inst) gm_destroy(c);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1483,6 +1497,9 @@ extern "C" int gm_batch_run(gm_batch *b, const gm_run_opts *o, gm_run_stats *st)
         h.nan_seen = 0;
         h.diag_valid = 0;
         h.stop_ratio = o->stop_ratio;
+        h.event = 0;
+        h.ev_dump = o->dt_dump;
+        h.ev_save = o->dt_save;
         long long ncyc = o->n_cycle > 0 ? o->n_cycle : c->p.nCycle;
         h.n_cycle = h.n_accept + ncyc;
         h.max_attempt = h.n_attempt + (o->max_attempts > 0 ? o->max_attempts : (1LL << 40));
@@ -1492,7 +1509,7 @@ extern "C" int gm_batch_run(gm_batch *b, const gm_run_opts *o, gm_run_stats *st)
     const int batch = std::max(1, o->batch);
     auto t0 = std::chrono::steady_clock::now();
     if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
-    if (o->use_graph) {
+    if (o->use_graph && !b->tm.on) {
         hipGraph_t gr;
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < batch; ++i) batch_enqueue_step(b);
@@ -1510,6 +1527,7 @@ extern "C" int gm_batch_run(gm_batch *b, const gm_run_opts *o, gm_run_stats *st)
         for (int i = 0; i < n; ++i)
             HIPCHK(hipMemcpyAsync(b->ctlh[i], b->inst[i]->ctl, sizeof(gmd::Ctl), hipMemcpyDeviceToHost, b->stream));
         HIPCHK(hipStreamSynchronize(b->stream));
+        b->tm.collect();
         bool all = true;
         for (int i = 0; i < n; ++i) all = all && b->ctlh[i]->done;
         if (all) break;
@@ -1534,8 +1552,118 @@ extern "C" int gm_batch_run(gm_batch *b, const gm_run_opts *o, gm_run_stats *st)
             s.stopped = h.done && !h.converged;
             s.nan_seen = h.nan_seen;
             s.seconds = std::chrono::duration<double>(t1 - t0).count();
+            s.event = h.event;
         }
     }
     if (nan) return fail(GM_ERR_NAN, "NaN in the error estimate of an ensemble member");
+    return GM_OK;
+}
+
+// Ensemble tile autotuner (no reference counterpart).  Per stage kernel the
+// candidates are tile policies for all members at once: each member takes
+// its fewest-lines (or fewest-points) rect within a workgroup bound of 512,
+// 768 or 1024 points (as far as the stage kernel is built for it), the
+// members then share the plane-step model's z-chunk length, and `reps`
+// launches of the whole ensemble's stage are timed on the current states.
+// The fastest policy is kept.  Stage launches alone write only slopes and
+// candidate positions, so the members' accepted states, t and dt stay put.
+extern "C" int gm_batch_autotune(gm_batch *b, int reps, double ms_best[6])
+{
+    if (!b) return fail(GM_ERR_ARG, "null ensemble");
+    reps = std::max(1, reps);
+    const int n = (int)b->inst.size();
+    int r;
+    for (gm_ctx *c : b->inst) HIPCHK(hipStreamSynchronize(c->stream));
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    gmd::FusedArgs *dargs = nullptr;
+    int *dstart = nullptr;
+    if ((r = dalloc(&dargs, (size_t)n)) || (r = dalloc(&dstart, (size_t)n + 1))) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return r;
+    }
+    std::vector<gmd::FusedArgs> A(n);
+    std::vector<int> st(n + 1);
+    std::vector<int> bTX(n), bTY(n), bCZ(n);
+    const int bounds[3] = {512, 768, 1024};
+    int rc = GM_OK;
+    for (int s = 0; s < 6 && rc == GM_OK; ++s) {
+        double best = 1e30;
+        for (int i = 0; i < n; ++i) { bTX[i] = b->inst[i]->TX[s]; bTY[i] = b->inst[i]->TY[s]; bCZ[i] = b->inst[i]->CZ[s]; }
+        for (int w = 0; w < 3 && rc == GM_OK; ++w) {
+            if (bounds[w] > gmd::max_rect32(s, b->press)) break;
+            for (int m = 0; m < 2 && rc == GM_OK; ++m) {
+                for (gm_ctx *c : b->inst) {
+                    int tx = c->TX[s], ty = c->TY[s];
+                    if (!best_rect(c, 0, bounds[w], m == 0, tx, ty)) { tx = c->TX[s]; ty = c->TY[s]; }
+                    set_tile_k(c, s, tx, ty, c->p.nz);
+                }
+                batch_chunks_k(b, s);
+                st[0] = 0;
+                for (int i = 0; i < n; ++i) {
+                    if ((rc = ensure_partials(b->inst[i]))) break;
+                    A[i] = fused_args(b->inst[i], s, 0, s == 0 ? 1 : 0, 0);
+                    st[i + 1] = st[i] + A[i].nblocks;
+                }
+                if (rc) break;
+                if (hipMemcpy(dargs, A.data(), n * sizeof(gmd::FusedArgs), hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(dstart, st.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+                    rc = fail(GM_ERR_HIP, "gm_batch_autotune: argument upload");
+                    break;
+                }
+                const int nt = batch_nt(b, s);
+                gmd::launch_fused32_batch(b->stream, s, b->press, nt, dargs, dstart, n, st[n]);   // warm
+                if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(GM_ERR_HIP, "event"); break; }
+                for (int q = 0; q < reps; ++q)
+                    gmd::launch_fused32_batch(b->stream, s, b->press, nt, dargs, dstart, n, st[n]);
+                if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = fail(GM_ERR_HIP, "gm_batch_autotune: launch");
+                    break;
+                }
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms / reps < best) {
+                    best = ms / reps;
+                    for (int i = 0; i < n; ++i) { bTX[i] = b->inst[i]->TX[s]; bTY[i] = b->inst[i]->TY[s]; bCZ[i] = b->inst[i]->CZ[s]; }
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i) set_tile_k(b->inst[i], s, bTX[i], bTY[i], bCZ[i]);
+        if (ms_best) ms_best[s] = best;
+    }
+    dfree(dargs);
+    dfree(dstart);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    for (gm_ctx *c : b->inst)
+        if ((r = ensure_partials(c))) return r;
+    return GM_OK;
+}
+
+// Event timing of the ensemble's launches (no reference counterpart): like
+// gm_timing_enable / gm_timing_read, per kind 0..5 the stage launches over
+// all members and 6 the controller launch.  Graph capture is off while on.
+extern "C" int gm_batch_timing_enable(gm_batch *b, int on)
+{
+    if (!b) return fail(GM_ERR_ARG, "null ensemble");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->tm.collect();
+    b->tm.on = on != 0;
+    b->tm.reset();
+    return GM_OK;
+}
+extern "C" int gm_batch_timing_read(gm_batch *b, double ms[GM_TIMING_KINDS], int64_t count[GM_TIMING_KINDS])
+{
+    if (!b) return fail(GM_ERR_ARG, "null ensemble");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->tm.collect();
+    for (int i = 0; i < GM_TIMING_KINDS; ++i) {
+        if (ms) ms[i] = b->tm.ms[i];
+        if (count) count[i] = b->tm.count[i];
+    }
     return GM_OK;
 }

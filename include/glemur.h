@@ -211,6 +211,10 @@ typedef struct gm_run_opts {
     double  stop_ratio;     /* stop when max|JxB| / max(|J||B|) <= stop_ratio (<=0: off) */
     int     batch;          /* steps enqueued between host checks (>=1)     */
     int     use_graph;      /* capture one batch in a hipGraph and replay it */
+    double  dt_dump, dt_save;  /* stop after an accepted step that is due a dump
+                               or a save by the reference's rule (src/main.cu:
+                               362-376): (t - dt, t] with the adapted dt holds a
+                               multiple of dt_dump or dt_save (<= 0: off)      */
 } gm_run_opts;
 
 typedef struct gm_run_stats {
@@ -221,6 +225,7 @@ typedef struct gm_run_stats {
     double  energy;         /* last evaluated B2                             */
     int     converged, stopped, nan_seen;
     double  seconds;        /* wall time of the run, host clock              */
+    int     event;          /* the last accepted step is due a dump or save  */
 } gm_run_stats;
 
 /* Many attempted steps without host synchronisation inside a batch: the
@@ -251,6 +256,16 @@ int gm_batch_destroy(gm_batch *b);
 int gm_batch_context(gm_batch *b, int i, gm_ctx **ctx);
 /* Like gm_run for every member at once; stats has n entries (or NULL). */
 int gm_batch_run(gm_batch *b, const gm_run_opts *opts, gm_run_stats *stats);
+/* Ensemble tile autotuner: per stage kernel, times `reps` launches of the
+ * whole ensemble for each candidate policy (every member's fewest-lines or
+ * fewest-points rect within 512, 768 or 1024 points, shared z-chunk
+ * length) on the current states and keeps the fastest; ms_best (6 entries,
+ * or NULL) receives the winners' mean times.  States, t and dt unchanged. */
+int gm_batch_autotune(gm_batch *b, int reps, double ms_best[6]);
+/* Event timing of the ensemble's launches, as gm_timing_enable/read: index
+ * 0..5 the stage launches (all members in one), 6 the controller launch. */
+int gm_batch_timing_enable(gm_batch *b, int on);
+int gm_batch_timing_read(gm_batch *b, double ms[7], int64_t count[7]);
 
 /* ---- learned warm start (no reference counterpart) -------------------- */
 /* A 3-level U-Net (3x3x3 convolutions with 32/64/128 channels, 3x3x3
@@ -315,8 +330,13 @@ int gm_timing_read(gm_ctx *ctx, double ms[GM_TIMING_KINDS], int64_t count[GM_TIM
 /* Bytes per grid point and stage the fused kernel must move (algorithmic). */
 double gm_algorithmic_bytes_per_point_stage(int precision);
 /* Plain device copy of `bytes`, `reps` times: mean kernel ms (the yardstick
- * for HBM byte counters; no reference counterpart). */
+ * for HBM byte counters; no reference counterpart).  Same as
+ * gm_hbm_probe(bytes, 0, reps, ms_mean). */
 int gm_copy_probe(int64_t bytes, int reps, double *ms_mean);
+/* HBM yardsticks (no reference counterpart): mode 0 copies `bytes`, mode 1
+ * reads `bytes`, mode 2 writes `bytes`; each lane keeps eight 16-byte words
+ * in flight.  Mean kernel ms over `reps` launches. */
+int gm_hbm_probe(int64_t bytes, int mode, int reps, double *ms_mean);
 /* Fused-kernel tile configuration (x, y tile and z chunk); 0 = automatic. */
 /* tx or ty <= 0 picks the tile automatically for about `cz` workgroups. */
 int gm_set_tile(gm_ctx *ctx, int tx, int ty, int cz);
@@ -324,6 +344,10 @@ int gm_get_tile(gm_ctx *ctx, int *tx, int *ty, int *cz, int *nblocks);
 /* Tile of one stage kernel (stage 0..5, -1 the fields-only pass); each stage
  * kernel has its own (gm_set_tile sets them all alike). */
 int gm_get_stage_tile(gm_ctx *ctx, int stage, int *tx, int *ty, int *cz, int *nblocks);
+/* Set the tile of one stage kernel (stage 0..5, -1 the fields-only pass):
+ * tx x ty output columns per workgroup, z chunks of cz planes.  The rect
+ * (tx+4)(ty+4) must fit the stage kernel's largest workgroup. */
+int gm_set_stage_tile(gm_ctx *ctx, int stage, int tx, int ty, int cz);
 /* Tile autotuner (no reference counterpart): for every stage kernel, times
  * `reps` launches of each candidate rect (the fewest-loads rect for each
  * workgroup size the kernel is built for, with one or two rounds of

@@ -136,3 +136,40 @@ def test_inertia_run_and_restart(require_gpu, tmp_path):
     assert [r[1:] for r in cont] == [r[1:] for r in want], (cont, want)
     out = run_cli(tmp_path, "-p", "32", ncycle=5, ok=False, **INERTIA)
     assert "inertia" in out
+
+
+NO_EXTRA = ", redConvex = f, redWedgeMin = f, redEpsilonStar = f, redErrB_1ez = f"
+
+
+@pytest.mark.parametrize("prec", ["64", "32"])
+def test_batched_rows_from_step_log(require_gpu, tmp_path, prec):
+    """Without a column that only the separate reductions give, the
+    executable runs the attempts in device batches (stopping only for dumps,
+    saves and the end) and takes each row's B2, JMax and JxB_B2Max from the
+    step log: the diagnostics recorded at the start of the attempt after an
+    accepted step.  With such a column it stops every nTs accepted steps and
+    reduces the fields.  Both runs take the same steps: the same rows (time
+    index, t, dt, maxDelta printed identically; the diagnostics, printed to
+    six digits, within one unit of the last: the stage kernels reduce in
+    another order, and the FP32 ones in FP32, what the fields pass reduces in
+    FP64) and the same dumps and save, byte for byte."""
+    a, b = tmp_path / "log", tmp_path / "red"
+    for d in (a, b):
+        os.makedirs(d)
+    out = run_cli(a, "-p", prec, ncycle=40, io_extra=NO_EXTRA)
+    assert "time stepping: " in out
+    run_cli(b, "-p", prec, ncycle=40)
+    ha, ra = ts_rows(a)
+    hb, rb = ts_rows(b)
+    assert len(ra) == len(rb) == 20
+    off = lambda h: 1 if h[0] == "#" else 0
+    for x, y in zip(ra, rb):
+        assert x[:4] == y[:4], (x, y)
+        for name in ("B2", "JMax", "JxB_B2Max"):
+            va, vb = float(x[ha.index(name) - off(ha)]), float(y[hb.index(name) - off(hb)])
+            assert abs(va - vb) <= 1.01e-5 * abs(vb), (name, va, vb)
+    da = sorted(f for f in os.listdir(a / "data") if f.endswith(".vtk"))
+    db = sorted(f for f in os.listdir(b / "data") if f.endswith(".vtk"))
+    assert da == db and "save.vtk" in da and any(f.startswith("dump") for f in da)
+    for f in da:
+        assert (a / "data" / f).read_bytes() == (b / "data" / f).read_bytes(), f

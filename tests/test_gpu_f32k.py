@@ -333,3 +333,24 @@ def test_medium_braid_f32k_tracks_fp64_over_3000_steps(g, medium):
           f"F32K vs FP64: energy {de:.2e}, criterion {dc:.2e}, B {dB:.2e}, J {dJ:.2e}, positions {dx:.2e}")
     assert de < 1e-5 and dc < 1e-3 and dB < 1e-5 and dJ < 1e-3 and dx < 1e-5
     assert np.all(np.diff(b["e"]) <= 0)
+
+
+def test_f32k_download_b0_and_compensation_word(g):
+    """An F32K context stores B0 as FP32 (x, y, z) triples and each position
+    as its FP32 rounding plus the FP32 rounding of the remainder.  Downloads
+    of an uploaded distorted state give back, bit for bit, float32(B0) for
+    GM_FIELD_B0, float32(xb - float32(xb)) for GM_FIELD_XB_LO and the sum of
+    the two words for GM_FIELD_XB."""
+    p = small_braid()
+    B0, xb = evolved_state(p, 3)
+    with g.Context(p, g.PREC_F32K) as c:
+        c.upload(B0, xb, 0.0, 0.01)
+        b0 = c.download(g.FIELD_B0)
+        lo = c.download(g.FIELD_XB_LO)
+        x = c.download(g.FIELD_XB)
+    hi = xb.astype(np.float32).astype(np.float64)
+    want_lo = (xb - hi).astype(np.float32).astype(np.float64)
+    assert np.array_equal(b0, B0.astype(np.float32).astype(np.float64))
+    assert np.array_equal(lo, want_lo)
+    assert np.array_equal(x, hi + want_lo)
+    assert np.max(np.abs(x - xb)) <= 1e-12 * np.max(np.abs(xb))
