@@ -151,6 +151,7 @@ _SIGS = {
     "gm_gradp": (C.c_int, [_CTX]),
     "gm_rk_kk": (C.c_int, [_CTX, C.c_int, C.c_double]),
     "gm_rk_xnewstar": (C.c_int, [_CTX, _D]),
+    "gm_rk_xnewstar_noalloc": (C.c_int, [_CTX, _D]),
     "gm_accept": (C.c_int, [_CTX]),
     "gm_reductions": (C.c_int, [_CTX, _P(Reduction)]),
     "gm_fields": (C.c_int, [_CTX]),
@@ -358,6 +359,13 @@ class Context:
     def rk_xnewstar(self):
         md = C.c_double()
         _check(_lib.gm_rk_xnewstar(self._h, C.byref(md)))
+        return md.value
+
+    def rk_xnewstar_noalloc(self):
+        """gm_rk_xnewstar_noalloc: the same results with the library's static
+        scratch instead of allocated memory."""
+        md = C.c_double()
+        _check(_lib.gm_rk_xnewstar_noalloc(self._h, C.byref(md)))
         return md.value
 
     def reductions(self):

@@ -86,6 +86,9 @@ void launch_u_diag(hipStream_t s, const Geo &g, const double *U, const double *d
 void launch_inner_aos_to_padded(hipStream_t s, const Geo &g, const double *src, double *dst);
 // Final NaN-blind max of n partials into out[0].
 void launch_max_final(hipStream_t s, const double *partials, int n, double *out);
+// the static scratch of the allocation-free xNewStar entry point
+long long noalloc_scratch_doubles();
+double *noalloc_scratch(int device);
 // Diagnostics over inner points from B, J (padded SoA) and det:
 // out[0] = B2 (incl. dx dy dz), out[1] = JMax, out[2] = JxB_B2Max,
 // out[3] = JxB_JBMax, out[4] = max|JxB|, out[5] = max |J||B|.  `scratch`

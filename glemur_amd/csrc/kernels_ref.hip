@@ -6,6 +6,9 @@
 // checks each piece of the arithmetic on its own.
 #include "gm_device.hpp"
 
+#include <mutex>
+#include <vector>
+
 namespace gmd {
 
 namespace {
@@ -818,7 +821,35 @@ __global__ void k_fill_f32(float *p, long long n, float v)
     if (a < n) p[a] = v;
 }
 
+// ------------------------------------------- allocation-free entry point
+// Scratch of gm_rk_xnewstar_noalloc: slot 0 the grid maximum, slot 1 the NaN
+// flag (an int), slots 8.. the per-block maxima of k_xnewstar (one per 256
+// inner points).  One copy per device, kept until the process ends; every
+// call on a device shares it, so calls that overlap on one device race.
+constexpr long long kNoAllocScratch = 1LL << 20;   // doubles: 8 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
 }  // namespace
+
+long long noalloc_scratch_doubles() { return kNoAllocScratch; }
+
+// Device address of `scratch` on `device`, which must be current: looked up
+// once per device by hipGetSymbolAddress (host code may not use the array's
+// name as a device address); nullptr if the lookup fails.
+double *noalloc_scratch(int device)
+{
+    static std::mutex mu;
+    static std::vector<double *> ptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) return nullptr;
+    if ((int)ptr.size() <= device) ptr.resize(device + 1, nullptr);
+    if (!ptr[device]) {
+        void *a = nullptr;
+        if (hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        ptr[device] = static_cast<double *>(a);
+    }
+    return ptr[device];
+}
 
 // ---------------------------------------------------------------- wrappers
 static long long ninner(const Geo &g) { return (long long)g.nx * g.ny * g.nz; }

@@ -730,6 +730,43 @@ extern "C" int gm_rk_xnewstar(gm_ctx *c, double *maxDelta)
     if (maxDelta) *maxDelta = c->red_host[0];
     return GM_OK;
 }
+// xNewStar + findDevMax like gm_rk_xnewstar, with every piece of scratch
+// (the per-block maxima, the NaN flag and the grid maximum) in the library's
+// static device array instead of memory the library allocates: no
+// hipMalloc, hipFree or allocating call.  At most noalloc_scratch_doubles()
+// - 8 block maxima (256 inner points each); calls that overlap on one
+// device race for the array.
+extern "C" int gm_rk_xnewstar_noalloc(gm_ctx *c, double *maxDelta)
+{
+    if (!c) return fail(GM_ERR_ARG, "null context");
+    if (int r = need_f64(c, "gm_rk_xnewstar_noalloc")) return r;
+    const long long nb = (c->N + 255) / 256;
+    if (nb + 8 > gmd::noalloc_scratch_doubles())
+        return fail(GM_ERR_ARG, "gm_rk_xnewstar_noalloc: the grid needs " + std::to_string(nb) +
+                                    " block maxima; the static scratch holds " +
+                                    std::to_string(gmd::noalloc_scratch_doubles() - 8) +
+                                    " (at most 256 of them per inner point count)");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != c->device) HIPCHK(hipSetDevice(c->device));
+    double *S = gmd::noalloc_scratch(c->device);
+    if (!S) return fail(GM_ERR_HIP, "gm_rk_xnewstar_noalloc: hipGetSymbolAddress of the scratch failed");
+    double *out = S, *parts = S + 8;
+    int *nan = reinterpret_cast<int *>(S + 1);
+    gmd::launch_fill_f64(c->stream, S, 2, 0.0);   // grid maximum and NaN flag start at zero
+    double *xnew = c->X64[(c->cur + 1) % 3];
+    // xb_new starts as xb (src/main.cu:209) so its ghost layers match
+    HIPCHK(hipMemcpyAsync(xnew, c->X64[c->cur], 3 * (size_t)c->P * 8, hipMemcpyDeviceToDevice, c->stream));
+    const bool in = c->p.inertia;
+    int n = gmd::launch_xnewstar(c->stream, c->geo, c->tab_dev, c->X64[c->cur], c->K64, xnew, parts, nan,
+                                 in ? c->U64[c->ucur] : nullptr, in ? c->KU64 : nullptr,
+                                 in ? c->U64[c->ucur ^ 1] : nullptr);
+    if (any_peri(c->p)) gmd::launch_set_grid_peri(c->stream, c->geo, xnew);
+    gmd::launch_max_final(c->stream, parts, n, out);
+    HIPCHK(hipMemcpyAsync(c->red_host, out, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (maxDelta) *maxDelta = c->red_host[0];
+    return GM_OK;
+}
 extern "C" int gm_accept(gm_ctx *c)
 {
     if (!c) return fail(GM_ERR_ARG, "null context");

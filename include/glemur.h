@@ -189,6 +189,14 @@ int gm_gradp(gm_ctx *ctx);                   /* gradP,      src/gradP.cu:160    
 int gm_rk_kk(gm_ctx *ctx, int n, double dt);
 /* xNewStar<<<>>> (src/rungeKutta.cu:251) + findDevMax (src/maths.cu:101). */
 int gm_rk_xnewstar(gm_ctx *ctx, double *maxDelta);
+/* gm_rk_xnewstar without any allocation: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * gm_rk_xnewstar; its scratch (per-block maxima, NaN flag, grid maximum) is
+ * a static device array of 2^20 doubles (8 MiB) per device, so grids of
+ * more than (2^20 - 8) * 256 inner points are rejected with GM_ERR_ARG.
+ * Calls that overlap on one device race for that array. */
+int gm_rk_xnewstar_noalloc(gm_ctx *ctx, double *maxDelta);
 /* xb <- xb_new (src/main.cu:324), done by exchanging buffers. */
 int gm_accept(gm_ctx *ctx);
 /* reductions (src/diagnostics.cu:12-134) over the fields B, J, detJac the
