@@ -33,9 +33,18 @@ for 2000 accepted F32K steps from its initial state; aggregate grid-point
 RK-stages per second over the attempts.
 
 time_to_relax: wall time from the initial state of the 129 x 129 x 385
-braid to max|JxB| / max(|J||B|) <= 0.03, cold and with the learned warm
+braid to max|JxB| / max(|J||B|) <= 0.023, cold and with the learned warm
 start (U-Net inference, up-sampling, checks and 50 settle attempts
-included), and the energies of the two states reached.
+included), and the energies of the two states reached.  0.023 is the
+tightest threshold both runs reach: the cold run's criterion bottoms out
+at 0.0228 (t = 17.9) on this grid, and the floor rises with resolution
+(0.0085 at 33x33x97 and 65x65x193, 0.023 at 129x129x385, >= 0.058 at
+257x257x769), set by a force concentration on the braid's axis, so the
+1e-3 the brief names is out of reach for this braid
+(glemur_amd/relax_study.py).
+
+cli: the glemur executable on the same braid in a run directory (device
+batches between time-series rows, its own stepping time).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -52,22 +61,38 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_braid129x129x385_hbm_traffic.md")
 
 
-def measured_traffic():
-    """{prec: (per-stage tiles, bytes per launch)} from the committed counter
-    summary; the tiles are (tx, ty, cz, workgroups) of stages 0..5."""
+TRAFFIC_C3 = os.path.join(ROOT, "profiles", "k_stage32_pressure_braid257x257x769_hbm_traffic.md")
+TRAFFIC_C4 = os.path.join(ROOT, "profiles", "k_stage32_batch_ensemble64_hbm_traffic.md")
+
+
+def profile_values(path):
+    """The `key value...` lines of a committed counter summary."""
     try:
-        lines = open(TRAFFIC_FILE).read().splitlines()
+        lines = open(path).read().splitlines()
     except OSError:
         return {}
-    tiles, tr = {}, {}
+    out = {}
     for l in lines:
         f = l.split()
-        if len(f) == 5 and f[0].startswith("tile_") and "_stage" in f[0]:
-            prec, st = f[0][5:].split("_stage")
-            tiles.setdefault(prec, {})[int(st)] = tuple(int(x) for x in f[1:])
-        elif len(f) == 2 and f[0].startswith("traffic_bytes_per_launch_"):
-            tr[f[0].rsplit("_", 1)[1]] = float(f[1])
-    return {k: ([tiles.get(k, {}).get(st) for st in range(6)], v) for k, v in tr.items()}
+        if len(f) >= 2 and (f[0].startswith("tile_") or f[0].startswith("traffic_") or f[0].startswith("launch_")):
+            try:
+                out[f[0]] = tuple(float(x) for x in f[1:])
+            except ValueError:
+                pass
+    return out
+
+
+def measured_traffic(path=TRAFFIC_FILE):
+    """{prec: (per-stage tiles, bytes per launch)} from a committed counter
+    summary; the tiles are (tx, ty, cz, workgroups) of stages 0..5."""
+    v = profile_values(path)
+    out = {}
+    for key, val in v.items():
+        if key.startswith("traffic_bytes_per_launch_"):
+            prec = key.rsplit("_", 1)[1]
+            tiles = [v.get(f"tile_{prec}_stage{st}") for st in range(6)]
+            out[prec] = ([tuple(int(x) for x in t) if t else None for t in tiles], val[0])
+    return out
 
 
 def traffic_for(traffic, prec, stage_tiles):
@@ -157,6 +182,15 @@ def ensemble(g, workloads, n, steps, autotune=True):
         B0, xb = g.init_state(p)
         m.upload(B0, xb, 0.0, 0.01)
     tuned = b.autotune(reps=2) if autotune else None
+    # launch configuration per stage: the largest workgroup a member's rect
+    # needs (at least 512 threads) and the members' workgroups together
+    launches = []
+    for st in range(6):
+        tl = [m.stage_tile(st) for m in b.members]
+        launches.append((max(512, max(threads32((t[0] + 4) * (t[1] + 4)) for t in tl)), sum(t[3] for t in tl)))
+    pv = profile_values(TRAFFIC_C4)
+    prof_launch = [tuple(int(x) for x in pv.get(f"launch_ensemble_stage{st}", ())) for st in range(6)]
+    traffic = pv.get("traffic_bytes_per_launch_ensemble", (None,))[0] if prof_launch == launches else None
     b.run(max_attempts=3, batch=3, n_cycle=10 ** 9)             # warm-up
     t0 = time.perf_counter()
     st = b.run(max_attempts=steps, batch=steps, n_cycle=10 ** 9)
@@ -177,7 +211,10 @@ def ensemble(g, workloads, n, steps, autotune=True):
             "unit": "grid-pt RK-stages/s", "attempts_total": int(att),
             "tiles": "ensemble autotuner (gm_batch_autotune)" if autotune else "default",
             "autotune_stage_ms": tuned,
-            "roofline": {"kernel": "k_stage32_batch (one launch per stage over all members)",
+            "roofline": {"kernel": "k_stage32_batch (one launch per stage over all members; workgroup size, "
+                                   "workgroups per stage: " + ", ".join(f"{a}x{b}" for a, b in launches) + ")",
+                         "traffic": traffic,
+                         "traffic_source": os.path.relpath(TRAFFIC_C4, ROOT) if traffic else None,
                          "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "unit": "GB/s",
                          "mean_launch_ms": mean_ms, "per_stage_ms": per_stage,
                          "controller_ms": ms[6] / max(1, cnt[6]),
@@ -236,6 +273,9 @@ def large_run(g, workloads, steps):
             "energy_first": first.energy if first else None, "energy": st.energy,
             "JxBMax_JBMax_first": first.ratio if first else None, "JxBMax_JBMax": st.ratio,
             "roofline": {"kernel": kernel_name(stage_tiles, True), "achieved": achieved,
+                         "traffic": traffic_for(measured_traffic(TRAFFIC_C3), "f32k", stage_tiles),
+                         "traffic_source": os.path.relpath(TRAFFIC_C3, ROOT)
+                         if traffic_for(measured_traffic(TRAFFIC_C3), "f32k", stage_tiles) else None,
                          "frac": achieved / HBM_PEAK_GBS, "unit": "GB/s", "mean_launch_ms": mean_ms,
                          "per_stage_ms": per_stage, "controller_ms": ms[6] / max(1, cnt[6]),
                          "algorithmic_bytes_per_launch": 80.0 * n_pts}}
@@ -368,7 +408,7 @@ def main():
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--relax-seconds", type=float, default=10.0)
     ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0: skip)")
-    ap.add_argument("--ttr", type=float, default=0.03,
+    ap.add_argument("--ttr", type=float, default=0.023,
                     help="time-to-relax threshold for the cold/warm-start comparison (0: skip)")
     ap.add_argument("--large-steps", type=int, default=2000,
                     help="accepted steps of the 257x257x769 pressure-on braid (0: skip)")

@@ -3,17 +3,24 @@
 // Same 2.5D march as the FP64 kernel (kernels_fused.hip): a workgroup owns
 // an x-y tile plus a 2-point halo (one thread per column) and a z chunk,
 // and per plane p it forms the stage position of plane p+1, issues the HBM
-// loads of plane p+2, evaluates B at plane p and J, force and slope at
-// plane p-1, with one barrier per plane.  What differs, for FP32:
-//   * a point's position (hi x, y, z + lo x | lo y, z) is one 16-byte and
-//     one 8-byte LDS word, B one 16-byte word: a neighbour costs two LDS
-//     reads instead of six;
+// loads of a plane further ahead, evaluates B at plane p and J, force and
+// slope at plane p-1, with one barrier per plane.  What differs, for FP32:
+//   * a point's position (hi x, y, z + small x | small y, z) is one 16-byte
+//     and one 8-byte LDS word, B one 16-byte word: a neighbour costs two
+//     LDS reads instead of six;
 //   * a column's own z-neighbours are read back from the LDS rings (4
-//     position planes, 3 B planes) instead of being queued in registers, so
-//     without pressure stages 0-4 and the fields pass fit 128 VGPRs
-//     (1024-thread workgroups, wide tiles such as 65 x 10 columns) and the
-//     final stage 168 (768 threads); with pressure stages 0-4 run in 768-
-//     and the final stage and the fields pass in 512-thread workgroups;
+//     position planes, 3 B planes) instead of being queued in registers;
+//   * the loads of a plane are issued two plane steps before its use (one
+//     for the kernels whose registers hold only one input slot), into two
+//     input slots that alternate with the march unrolled by two.  Every
+//     lane issues every load and store of a step (planes and points clamped
+//     into range, stores of lanes without an output dropped at offset
+//     2^31), so the compiler's wait before a step consumes its slot covers
+//     that slot's loads only;
+//   * workgroups: 512 threads (two per CU) for stage 0 of a standalone
+//     context, 1024 for stages 1-4 and the fields pass, 768 for the final
+//     stage; with pressure 768 for stages 0-4 and 512 for the final stage
+//     and the fields pass (the host picks the smallest that holds a rect);
 //   * positions (hi, lo), B0 and the slopes are stored as (x, y, z)
 //     triples, one 12-byte load or store per point and array: a rect row
 //     of an array is one run of 12 (TX + 4) bytes, so its loads touch about
@@ -23,11 +30,12 @@
 //     the planes the previous stage wrote last (still in the memory-side
 //     cache) are the first the next stage reads; reversal maps virtual
 //     plane q to nz + 1 - q and negates z differences, which is exact;
-//   * ghost faces x = const and y = const take their tangential Jacobian
-//     columns from the ghost layer itself (in LDS) and the outer ghost from
-//     x_ghost +- h n, with no table loads inside the march; the table of
-//     precomputed face constants is used for the z faces (first and last
-//     plane only) and for grids with an initial boundary distortion;
+//   * line-tied faces x = const and y = const go through the interior
+//     formula: the column beyond the face holds the outer ghost x_g -+ h n
+//     of its face point, the tangential neighbours are the ghost layer, and
+//     B . n = B0 . n is set afterwards; the table of precomputed face
+//     constants is used for the z faces (first and last plane only) and for
+//     every face of a grid with an initial boundary distortion;
 //   * the final stage carries, per plane, sum_m (c_m - b5_m) k_m and
 //     sum_m (c_m - c*_m) k_m over the five stored slopes and adds
 //     c_5 k_5 at the point: x_new = x_5 + sum_m (c_m - b5_m) k_m by a
@@ -146,10 +154,17 @@ __device__ __forceinline__ BufF buf_of(const float *p, long long nfloats)
     return BufF{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0,
                                                   (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL), 0x00020000)};
 }
-template <int NT>
+// Final stage: the per-plane sums (c - b5) k and (c - c*) k wait two plane
+// steps for the point's J.  With pressure they wait in LDS (a thread's own
+// two-slot ring), which takes the pressure variant from 185 VGPRs in
+// 512-thread workgroups to 168 in 768-thread ones (257 x 257 x 769: 2466
+// -> 1952 us); without pressure a three-plane register queue is as fast
+constexpr bool final_lds(bool press) { return press; }
+template <int NT, int STAGE = 0, bool PRESS = false>
 constexpr size_t lds_bytes32()
 {
-    return (size_t)kNXL * NT * (sizeof(float4) + sizeof(float2)) + (size_t)kNBL * NT * sizeof(float4);
+    return (size_t)kNXL * NT * (sizeof(float4) + sizeof(float2)) + (size_t)kNBL * NT * sizeof(float4) +
+           (STAGE == 5 && final_lds(PRESS) ? (size_t)2 * NT * (sizeof(float4) + sizeof(float2)) : 0);
 }
 
 // det of an FP32 Jacobian in FP64 by cofactors (products of two floats are
@@ -205,12 +220,16 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     // the (runtime-off) branch: without it their stage 0 takes 30 more VGPRs
     constexpr bool EMIT = STAGE == 6 || PRESS;
     constexpr int kPF = pf_depth(STAGE, PRESS);
+    constexpr bool kFinalLds = FINAL && final_lds(PRESS);   // final-stage sums in LDS
 
     extern __shared__ __align__(16) unsigned char smem[];
     Lds<NT> L;
     L.xa = reinterpret_cast<float4 *>(smem);
     L.xb = reinterpret_cast<float2 *>(L.xa + kNXL * NT);
     L.bl = reinterpret_cast<float4 *>(L.xb + kNXL * NT);
+    float4 *fqa = L.bl + kNBL * NT;                          // kFinalLds: [2][NT]
+    float2 *fqb = reinterpret_cast<float2 *>(fqa + 2 * NT);  // sums (c - b5) k | (c - c*) k
+    (void)fqa; (void)fqb;
     __shared__ double red[7][NT / 64];
 
     const Geo &g = A.g;
@@ -589,17 +608,27 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 const int va = 12 * (int)a;
                 Pt3 xc;
                 L.ldx(pj, t, xc);
+                float fg0[3], fe0[3];   // the sums of plane pj
+                if constexpr (kFinalLds) {
+                    const float4 qa = fqa[(pj & 1) * NT + t];
+                    const float2 qb = fqb[(pj & 1) * NT + t];
+                    fg0[0] = qa.x; fg0[1] = qa.y; fg0[2] = qa.z;
+                    fe0[0] = qa.w; fe0[1] = qb.x; fe0[2] = qb.y;
+                } else {
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) { fg0[l] = fg[0][l]; fe0[l] = fe[0][l]; }
+                }
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
                     float hi = xc.h[l], lo;
                     if (l < 2 || g.zUpdate) {
-                        float d = xc.s[l] + (fg[0][l] + (float)c_tab32.c[5] * kv[l]);
+                        float d = xc.s[l] + (fg0[l] + (float)c_tab32.c[5] * kv[l]);
                         float s, e;
                         two_sum(hi, d, s, e);
                         float hi2 = s + e;
                         lo = e - (hi2 - s);
                         hi = hi2;
-                        double dd = fabs((double)fe[0][l] + c_tab32.ce[5] * (double)kv[l]);
+                        double dd = fabs((double)fe0[l] + c_tab32.ce[5] * (double)kv[l]);
                         if (dd != dd) nan_seen = 1;
                         dmax = nb_max(dmax, dd);
                     } else {
@@ -616,6 +645,12 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         } else {
             bXhd.st3(hn, st_off);
             bXld.st3(ln, st_off);
+            // the sums of plane p+1, formed at the top of this step, into the
+            // slot plane p-1's were just read from
+            if (kFinalLds && ldc && plane_ok(p + 1)) {
+                fqa[((p + 1) & 1) * NT + t] = make_float4(fg[2][0], fg[2][1], fg[2][2], fe[2][0]);
+                fqb[((p + 1) & 1) * NT + t] = make_float2(fe[2][1], fe[2][2]);
+            }
         }
 
         // one barrier per plane
@@ -628,7 +663,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
             for (int l = 0; l < 3; ++l) drq[0][a2][l] = drq[1][a2][l];
-        if constexpr (FINAL) {
+        if constexpr (FINAL && !final_lds(PRESS)) {
 #pragma unroll
             for (int l = 0; l < 3; ++l) {
                 fg[0][l] = fg[1][l]; fg[1][l] = fg[2][l];
@@ -726,7 +761,7 @@ template <int NT, int STAGE, bool PRESS>
 void launch_one32(hipStream_t s, const FusedArgs &A)
 {
     static bool attr = false;
-    constexpr size_t lds = lds_bytes32<NT>();
+    constexpr size_t lds = lds_bytes32<NT, STAGE, PRESS>();
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_stage32<NT, STAGE, PRESS>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -769,13 +804,12 @@ void launch_1024(hipStream_t s, const FusedArgs &A)
 
 int threads32(int rect) { return rect <= 256 ? 256 : rect <= 384 ? 384 : rect <= 512 ? 512 : rect <= 768 ? 768 : 1024; }
 
-// 768-thread workgroups (12 waves, 168 VGPRs, three waves per SIMD) for
-// the kernels that fit that budget without spilling: all but the pressure
-// variants of the final stage and of the fields pass.  The final stage
-// without pressure: 266 instead of 272 us at 129 x 129 x 385; the pressure
-// stages 0-4 at 257 x 257 x 769: 4-11% faster than in 512-thread
-// workgroups
-constexpr bool fits768(int stage, bool press) { return !(press && (stage == 5 || stage == 6)); }
+// 768-thread workgroups (12 waves, 168 VGPRs, three waves per SIMD): every
+// stage kernel fits that budget without spilling (the pressure final stage
+// with its per-plane sums in LDS).  The final stage without pressure: 266
+// instead of 272 us at 129 x 129 x 385; the pressure stages 0-4 at
+// 257 x 257 x 769: 4-11% faster than in 512-thread workgroups
+constexpr bool fits768(int, bool) { return true; }
 
 int max_rect32(int stage, int press)
 {
@@ -797,7 +831,7 @@ template <int NT, int STAGE, bool PRESS>
 void launch_batch_one(hipStream_t s, const FusedArgs *args, const int *start, int n, int total)
 {
     static bool attr = false;
-    constexpr size_t lds = lds_bytes32<NT>();
+    constexpr size_t lds = lds_bytes32<NT, STAGE, PRESS>();
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_stage32_batch<NT, STAGE, PRESS>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

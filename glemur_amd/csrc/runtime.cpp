@@ -155,21 +155,17 @@ double rect_lines(int tlx, int tly, int rw, int rh)
 //    tiles take stages 0-4 4-7% faster than 26 x 26;
 //  - FP64: the fewest loaded points per output point (its register-queue
 //    march is 20% slower on the wide rects);
-//  - F32K stage 0 without pressure, outside ensembles: the fewest loaded
-//    points among rects of up to 512 points, two workgroups per CU.  It
-//    waits on its barriers more than any other stage, and two independent
-//    workgroups per CU hide that: at 129 x 129 x 385 on 26 x 13 tiles it
-//    takes 134 instead of 146 us on 65 x 10 (ensemble members lose 1%
-//    with it, on their smaller grids, and keep the line model);
 // then the z chunks of best_chunks with `target` x the kernel's workgroups
-// per CU resident at a time.
+// per CU resident at a time.  (Stage 0 took 512-point rects, two
+// workgroups per CU, while its loads were issued one plane step ahead; with
+// two steps it is faster on the wide rects: 98 -> 92 us at 129 x 129 x 385.)
 void auto_tiles(gm_ctx *c, int target, bool member)
 {
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     const bool lines = c->prec == GM_PREC_F32K;
+    (void)member;   // ensemble members take the same rule
     for (int k = 0; k < 7; ++k) {
-        const bool two = lines && k == 0 && !c->p.pressure && !member;
-        const int cap = two ? 512 : stage_max_rect(c, k);
+        const int cap = stage_max_rect(c, k);
         int bx = 4, by = 2;
         double best = 1e30;
         for (int ty = 2; ty <= 96; ++ty)
@@ -177,7 +173,7 @@ void auto_tiles(gm_ctx *c, int target, bool member)
                 int tlx = (nx + txm - 1) / txm, tly = (ny + ty - 1) / ty;
                 int tx = (nx + tlx - 1) / tlx, tyy = (ny + tly - 1) / tly;
                 if ((tx + 4) * (tyy + 4) > cap) continue;
-                const double cost = (lines && !two ? rect_lines(tlx, tly, tx + 4, tyy + 4)
+                const double cost = (lines ? rect_lines(tlx, tly, tx + 4, tyy + 4)
                                            : (double)tlx * tly * (tx + 4) * (tyy + 4)) / ((double)nx * ny);
                 if (cost < best - 1e-12) { best = cost; bx = tx; by = tyy; }
             }

@@ -49,11 +49,12 @@ def test_autotune_keeps_state_and_trajectory(g, prec):
 @pytest.mark.parametrize("press", [False, True])
 def test_f32k_workgroup_sizes_give_the_same_trajectory(g, press):
     """F32K stage kernels at every workgroup size they are built for: on a
-    61 x 57 x 40 grid the default tiles run stage 0 in 512-thread, stages
-    1-4 in 1024-thread (with pressure: stages 0-4 in 768-thread) and the
-    final stage in 768-thread (with pressure: 512-thread) workgroups; forced
-    tiles run all stages in 256- and 512-thread workgroups.  The positions (hi and lo words), t and dt after
-    10 attempts agree bit for bit."""
+    61 x 57 x 40 grid the default tiles run stages 0-4 in 1024-thread and
+    the final stage in 768-thread workgroups (with pressure: every stage in
+    768-thread workgroups, the final stage with its per-plane sums in LDS);
+    forced tiles run all stages in 256- and 512-thread workgroups.  The
+    positions (hi and lo words), t and dt after 10 attempts agree bit for
+    bit."""
     p = small_braid(61, 57, 40, pressure=press, beta=0.05)
     p.nCycle = 10 ** 6
     B0, xb = evolved_state(p, 1)
@@ -69,9 +70,9 @@ def test_f32k_workgroup_sizes_give_the_same_trajectory(g, press):
             res.append((c.download(g.FIELD_XB), c.download(g.FIELD_XB_LO), c.time()))
     print("rects per stage", sizes)
     if press:
-        assert all(512 < r <= 768 for r in sizes[0][:5]) and 384 < sizes[0][5] <= 512
+        assert all(512 < r <= 768 for r in sizes[0])
     else:
-        assert 384 < sizes[0][0] <= 512 and all(768 < r <= 1024 for r in sizes[0][1:5]) and 512 < sizes[0][5] <= 768
+        assert all(768 < r <= 1024 for r in sizes[0][:5]) and 512 < sizes[0][5] <= 768
     assert all(r <= 256 for r in sizes[1]) and all(384 < r <= 512 for r in sizes[2])
     for r in res[1:]:
         assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1]) and r[2] == res[0][2]

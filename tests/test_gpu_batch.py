@@ -100,9 +100,9 @@ def test_default_tiles_mix_workgroup_sizes(g):
     """With their default tiles the members below need 1024-, 768- and
     512-thread workgroups for stages 0-4; each stage launch runs at the
     largest (members with smaller rects leave threads idle).  Every member
-    equals its standalone run with default tiles: the same tiles for stages
-    1-5 (a standalone stage 0 takes 512-point rects), other z chunks; tiles
-    and chunks change the order of the energy sum only."""
+    equals its standalone run with default tiles: the same tiles, other z
+    chunks (members share one chunk length per stage); chunks change the
+    order of the energy sum only."""
     ps = [small_braid(52, 52, 24), small_braid(33, 30, 21), small_braid(19, 17, 23, epsilonProf=True)]
     for p in ps:
         p.nCycle = 10 ** 6
@@ -117,7 +117,7 @@ def test_default_tiles_mix_workgroup_sizes(g):
         got = [(m.download(g.FIELD_XB), m.time(), m.step_log(20)) for m in b.members]
     for i, (p, (B0, xb)) in enumerate(zip(ps, states)):
         with g.Context(p, g.PREC_F32K) as c:
-            assert [c.stage_tile(s)[:2] for s in range(1, 6)] == tiles[i][1:]
+            assert [c.stage_tile(s)[:2] for s in range(6)] == tiles[i]
             c.upload(B0, xb, 0.0, 0.01)
             st = c.run(max_attempts=20, batch=5)
             x, t, log = c.download(g.FIELD_XB), c.time(), c.step_log(20)
