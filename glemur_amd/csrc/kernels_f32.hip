@@ -160,11 +160,21 @@ __device__ __forceinline__ BufF buf_of(const float *p, long long nfloats)
 // 512-thread workgroups to 168 in 768-thread ones (257 x 257 x 769: 2466
 // -> 1952 us); without pressure a three-plane register queue is as fast
 constexpr bool final_lds(bool press) { return press; }
+// The other pressure kernels put the y and z central differences of each
+// B-ring point in LDS for one plane: gradP's mixed second differences at a
+// tile point are differences of them between its in-plane neighbours,
+// (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b)) = dr_b(+a) - dr_b(-a), the same
+// floating-point operations on the same values as from the positions, with
+// 8 LDS reads instead of 24
+constexpr bool dr_lds(int stage, bool press) { return press && stage != 5; }
 template <int NT, int STAGE = 0, bool PRESS = false>
 constexpr size_t lds_bytes32()
 {
+    // positions, B, and one two-slot ring of a float4 and a float2 per
+    // thread for the final stage's sums or the pressure kernels' differences
     return (size_t)kNXL * NT * (sizeof(float4) + sizeof(float2)) + (size_t)kNBL * NT * sizeof(float4) +
-           (STAGE == 5 && final_lds(PRESS) ? (size_t)2 * NT * (sizeof(float4) + sizeof(float2)) : 0);
+           ((STAGE == 5 && final_lds(PRESS)) || dr_lds(STAGE, PRESS)
+                ? (size_t)2 * NT * (sizeof(float4) + sizeof(float2)) : 0);
 }
 
 // det of an FP32 Jacobian in FP64 by cofactors (products of two floats are
@@ -221,14 +231,17 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     constexpr bool EMIT = STAGE == 6 || PRESS;
     constexpr int kPF = pf_depth(STAGE, PRESS);
     constexpr bool kFinalLds = FINAL && final_lds(PRESS);   // final-stage sums in LDS
+    constexpr bool kDrLds = dr_lds(STAGE, PRESS);           // y, z differences in LDS
 
     extern __shared__ __align__(16) unsigned char smem[];
     Lds<NT> L;
     L.xa = reinterpret_cast<float4 *>(smem);
     L.xb = reinterpret_cast<float2 *>(L.xa + kNXL * NT);
     L.bl = reinterpret_cast<float4 *>(L.xb + kNXL * NT);
-    float4 *fqa = L.bl + kNBL * NT;                          // kFinalLds: [2][NT]
-    float2 *fqb = reinterpret_cast<float2 *>(fqa + 2 * NT);  // sums (c - b5) k | (c - c*) k
+    // two-slot ring [2][NT]: the final stage's sums (c - b5) k | (c - c*) k
+    // (kFinalLds), or dr_y | dr_z of a plane (kDrLds)
+    float4 *fqa = L.bl + kNBL * NT;
+    float2 *fqb = reinterpret_cast<float2 *>(fqa + 2 * NT);
     (void)fqa; (void)fqb;
     __shared__ double red[7][NT / 64];
 
@@ -445,6 +458,26 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 ghost_b(g, G, face, side, xin, Bd, detv);
 #pragma unroll
                 for (int l = 0; l < 3; ++l) Bv[l] = (float)Bd[l];
+                if constexpr (kDrLds) {
+                    // x and y faces from the table: their y and z differences
+                    // still feed the mixed terms of the tile points beside them
+                    if (!zg) {
+                        Pt3 a, b;
+                        L.ldx(p, t + RW, a);
+                        L.ldx(p, t - RW, b);
+#pragma unroll
+                        for (int l = 0; l < 3; ++l) {
+                            drq[1][1][l] = dif(a, b, l);
+                            drq[1][2][l] = zs * dif(xzp, xzm, l);
+                        }
+                    }
+                }
+            }
+            if constexpr (kDrLds) {
+                if (!zg) {
+                    fqa[(p & 1) * NT + t] = make_float4(drq[1][1][0], drq[1][1][1], drq[1][1][2], drq[1][2][0]);
+                    fqb[(p & 1) * NT + t] = make_float2(drq[1][2][1], drq[1][2][2]);
+                }
             }
             if (EMIT && A.emit && (gx || gy || zg)) {
                 int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
@@ -501,23 +534,23 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 for (int l = 0; l < 3; ++l) db[2][l] = zs * (bp[l] - bm[l]);
             }
             const float (&dr)[3][3] = drq[0];
+            // cofactor vectors C_a = dr[a+1] x dr[a+2]: the rows of
+            // det(dr) dr^-1 (dr's columns are the three differences)
+            float C[3][3];
+#pragma unroll
+            for (int a0 = 0; a0 < 3; ++a0) cross3(dr[(a0 + 1) % 3], dr[(a0 + 2) % 3], C[a0]);
             float Jv[3] = {0.f, 0.f, 0.f}, kv[3];
 #pragma unroll
             for (int a0 = 0; a0 < 3; ++a0) {
-                const int a1 = (a0 + 1) % 3, a2 = (a0 + 2) % 3;
-                float C[3];
-                cross3(dr[a1], dr[a2], C);
                 // Jv += C x db[a0], two fused products per component
-                Jv[0] = fmaf(C[1], db[a0][2], fmaf(-C[2], db[a0][1], Jv[0]));
-                Jv[1] = fmaf(C[2], db[a0][0], fmaf(-C[0], db[a0][2], Jv[1]));
-                Jv[2] = fmaf(C[0], db[a0][1], fmaf(-C[1], db[a0][0], Jv[2]));
+                Jv[0] = fmaf(C[a0][1], db[a0][2], fmaf(-C[a0][2], db[a0][1], Jv[0]));
+                Jv[1] = fmaf(C[a0][2], db[a0][0], fmaf(-C[a0][0], db[a0][2], Jv[1]));
+                Jv[2] = fmaf(C[a0][0], db[a0][1], fmaf(-C[a0][1], db[a0][0], Jv[2]));
             }
-            {
-                // det[dx dy dz] = 8 hx hy hz det(jac), det(jac) from FP64
-                const float rD = __builtin_amdgcn_rcpf((float)(dq[0] * (8.0 * g.dx * g.dy * g.dz)));
+            // det[dx dy dz] = 8 hx hy hz det(jac), det(jac) from FP64
+            const float rD = __builtin_amdgcn_rcpf((float)(dq[0] * (8.0 * g.dx * g.dy * g.dz)));
 #pragma unroll
-                for (int l = 0; l < 3; ++l) Jv[l] *= rD;
-            }
+            for (int l = 0; l < 3; ++l) Jv[l] *= rD;
             kv[0] = Jv[1] * Bc[2] - Jv[2] * Bc[1];
             kv[1] = Jv[2] * Bc[0] - Jv[0] * Bc[2];
             kv[2] = Jv[0] * Bc[1] - Jv[1] * Bc[0];
@@ -527,9 +560,19 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             float gv[3] = {0, 0, 0};
             const double detv = dq[0];
             if constexpr (PRESS) {
-                // gradP from second differences (src/gradP.cu:93-154)
-                float jac[3][3], d2[3][3][3];
-                const float h1[3] = {(float)g.dx1, (float)g.dy1, (float)g.dz1};
+                // gradP of src/gradP.cu:93-154,
+                //   g_c = -beta det1 sum_{a,b,l} d2[a][b][l] jac1[b][a] jac1[l][c]
+                // (a a component, b and l axes, jac1 = jac^-1, det1 = 1/det(jac)),
+                // with the spacings cancelled: jac = dr diag(1/2h), so
+                // jac1 = diag(2h) C / det(dr), and the second derivatives are
+                // the raw second differences over h_b^2 (diagonal) or 4 h_b h_l
+                // (mixed).  Hence
+                //   g = -beta det1 / det(dr)^2 sum_l N_l C_l,
+                //   N_l = 4 (dd_l . C_l) + sum_{b != l} (dm_bl . C_b),
+                // dd_l = x(+l) - 2x + x(-l) and dm_bl = dr_b(+l) - dr_b(-l):
+                // 45 operations on top of the differences instead of about 190
+                // for the inverse Jacobian and the 27-term sum
+                float dd[3][3], mxy[3], mxz[3], myz[3];
                 Pt3 xc;
                 L.ldx(pj, t, xc);
                 const int vq[6] = {pj, pj, pj, pj, pj - 1, pj + 1};
@@ -540,29 +583,49 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     L.ldx(vq[2 * a2], vt[2 * a2], xm);
                     L.ldx(vq[2 * a2 + 1], vt[2 * a2 + 1], xp);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) {
-                        jac[l][a2] = dr[a2][l] * 0.5f * h1[a2];
-                        d2[l][a2][a2] = (dif(xp, xc, l) - dif(xc, xm, l)) * h1[a2] * h1[a2];
-                    }
+                    for (int l = 0; l < 3; ++l) dd[a2][l] = dif(xp, xc, l) - dif(xc, xm, l);
                 }
                 // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b))
-                Pt3 pa, pb, pc, pd;
-                // xy
-                L.ldx(pj, t + 1 + RW, pa); L.ldx(pj, t + 1 - RW, pb);
-                L.ldx(pj, t - 1 + RW, pc); L.ldx(pj, t - 1 - RW, pd);
+                if constexpr (kDrLds) {
+                    // = dr_b(+a) - dr_b(-a) from the ring; dr_z carries the
+                    // march sign zs already (exact: zs = -+1)
+                    const int sl = (pj & 1) * NT;
+                    const float4 ea = fqa[sl + t + 1], wa = fqa[sl + t - 1], na = fqa[sl + t + RW], sa = fqa[sl + t - RW];
+                    const float2 eb = fqb[sl + t + 1], wb = fqb[sl + t - 1], nb = fqb[sl + t + RW], sb = fqb[sl + t - RW];
+                    const float dyE[3] = {ea.x, ea.y, ea.z}, dyW[3] = {wa.x, wa.y, wa.z};
+                    const float dzE[3] = {ea.w, eb.x, eb.y}, dzW[3] = {wa.w, wb.x, wb.y};
+                    const float dzN[3] = {na.w, nb.x, nb.y}, dzS[3] = {sa.w, sb.x, sb.y};
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d2[l][0][1] = d2[l][1][0] = (dif(pa, pb, l) - dif(pc, pd, l)) * h1[0] * h1[1] * 0.25f;
-                // xz
-                L.ldx(pj + 1, t + 1, pa); L.ldx(pj - 1, t + 1, pb);
-                L.ldx(pj + 1, t - 1, pc); L.ldx(pj - 1, t - 1, pd);
+                    for (int l = 0; l < 3; ++l) {
+                        mxy[l] = dyE[l] - dyW[l];
+                        mxz[l] = dzE[l] - dzW[l];
+                        myz[l] = dzN[l] - dzS[l];
+                    }
+                } else {
+                    Pt3 pa, pb, pc, pd;
+                    // xy
+                    L.ldx(pj, t + 1 + RW, pa); L.ldx(pj, t + 1 - RW, pb);
+                    L.ldx(pj, t - 1 + RW, pc); L.ldx(pj, t - 1 - RW, pd);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d2[l][0][2] = d2[l][2][0] = zs * (dif(pa, pb, l) - dif(pc, pd, l)) * h1[0] * h1[2] * 0.25f;
-                // yz
-                L.ldx(pj + 1, t + RW, pa); L.ldx(pj - 1, t + RW, pb);
-                L.ldx(pj + 1, t - RW, pc); L.ldx(pj - 1, t - RW, pd);
+                    for (int l = 0; l < 3; ++l) mxy[l] = dif(pa, pb, l) - dif(pc, pd, l);
+                    // xz
+                    L.ldx(pj + 1, t + 1, pa); L.ldx(pj - 1, t + 1, pb);
+                    L.ldx(pj + 1, t - 1, pc); L.ldx(pj - 1, t - 1, pd);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d2[l][1][2] = d2[l][2][1] = zs * (dif(pa, pb, l) - dif(pc, pd, l)) * h1[1] * h1[2] * 0.25f;
-                gradp_core(jac, d2, __builtin_amdgcn_rcpf((float)detv), (float)g.beta, gv);
+                    for (int l = 0; l < 3; ++l) mxz[l] = zs * (dif(pa, pb, l) - dif(pc, pd, l));
+                    // yz
+                    L.ldx(pj + 1, t + RW, pa); L.ldx(pj - 1, t + RW, pb);
+                    L.ldx(pj + 1, t - RW, pc); L.ldx(pj - 1, t - RW, pd);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) myz[l] = zs * (dif(pa, pb, l) - dif(pc, pd, l));
+                }
+                auto dot = [](const float u[3], const float v[3]) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+                const float N0 = 4.f * dot(dd[0], C[0]) + dot(mxy, C[1]) + dot(mxz, C[2]);
+                const float N1 = 4.f * dot(dd[1], C[1]) + dot(mxy, C[0]) + dot(myz, C[2]);
+                const float N2 = 4.f * dot(dd[2], C[2]) + dot(mxz, C[0]) + dot(myz, C[1]);
+                const float f = -(float)g.beta * __builtin_amdgcn_rcpf((float)detv) * (rD * rD);
+#pragma unroll
+                for (int l = 0; l < 3; ++l) gv[l] = (N0 * C[0][l] + N1 * C[1][l] + N2 * C[2][l]) * f;
             }
 #pragma unroll
             for (int l = 0; l < 3; ++l) {

@@ -32,17 +32,30 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-@pytest.mark.parametrize("case", [{}, {"epsilonProf": True}, {"pressure": True, "beta": 0.05}])
+SHEAR = {"initDist": b"initShearX", "initShearA": 0.05, "initShearB": 0.02}
+SHIFT = {"initDist": b"centerShift", "initShearA": 0.05}
+
+
+@pytest.mark.parametrize("case", [{}, {"epsilonProf": True}, {"pressure": True, "beta": 0.05},
+                                  {"pressure": True, "beta": 0.05, **SHEAR},
+                                  {"pressure": True, "beta": 0.05, "tile": (7, 6, 5), **SHIFT}])
 def test_f32k_fields_accuracy(g, case):
     """B, det and J of the FP32 stage kernel against the FP64 oracle on a
     distorted grid.  B and det carry FP32 rounding (max-norm relative
     2e-6); J, a difference quotient of B over the cell, to 5e-6 of max|J|;
-    gradP, from second differences, to 2e-5 of max|gradP|."""
+    gradP, from second differences, to 2e-5 of max|gradP|.  The last two
+    cases start from an initially distorted grid, whose x and y faces take
+    their ghost values from the precomputed table (src/boundaries.cu:11-35),
+    one of them on several tiles."""
+    case = dict(case)
+    tile = case.pop("tile", None)
     p = small_braid(**case)
     B0, xb = evolved_state(p, 4)
     sim = oracle_sim(p, B0, xb)
     sim.fields()
     with g.Context(p, g.PREC_F32K) as ctx:
+        if tile:
+            ctx.set_tile(*tile)
         ctx.upload(B0, xb, 0.0, 0.01)
         ctx.fields()
         m = inner_mask(p) | face_mask(p)
@@ -298,15 +311,18 @@ def test_large_pressure_braid_f32k_tracks_fp64(g):
     assert np.all(np.diff(eb) <= 0) and eb[-1] < eb[0]
 
 
-def test_medium_braid_f32k_tracks_fp64_over_3000_steps(g, medium):
-    """The 129 x 129 x 385 braid relaxed for 3000 steps at a fixed dt of
-    2e-3 (t = 6, every attempt accepted in both precisions) in FP64 and in
-    F32K.  At the end: energy within 1e-5 relative (the accuracy target of
-    the compensated path), the relaxation criterion within 1e-3, B within
-    1e-5 and J within 1e-3 of their maxima, positions within 1e-5 of the
-    total displacement; the F32K energy never rises."""
+def test_medium_braid_f32k_tracks_fp64_to_the_relaxed_state(g, medium):
+    """The 129 x 129 x 385 braid relaxed at a fixed dt of 2e-3 for 9000
+    steps (every attempt accepted in both precisions) in FP64 and in F32K,
+    to t = 18, where max|JxB|/max(|J||B|) reaches its floor on this grid
+    (about 0.023; it rises again afterwards as the current layer on the
+    axis sharpens), so the end state is the relaxed state.  At the end:
+    energy within 1e-5 relative (the accuracy target of the compensated
+    path), the relaxation criterion within 1e-3, B within 1e-5 and J within
+    1e-3 of their maxima, positions within 1e-5 of the total displacement;
+    the F32K energy never rises."""
     p, B0, xb = medium
-    n, dt = 3000, 2e-3
+    n, dt = 9000, 2e-3
     out = {}
     for prec in ("F64", "F32K"):
         with g.Context(p, getattr(g, "PREC_" + prec)) as c:
@@ -333,6 +349,7 @@ def test_medium_braid_f32k_tracks_fp64_over_3000_steps(g, medium):
           f"F32K vs FP64: energy {de:.2e}, criterion {dc:.2e}, B {dB:.2e}, J {dJ:.2e}, positions {dx:.2e}")
     assert de < 1e-5 and dc < 1e-3 and dB < 1e-5 and dJ < 1e-3 and dx < 1e-5
     assert np.all(np.diff(b["e"]) <= 0)
+    assert a["crit"] < 0.03 and abs(a["t"] - n * dt) < 1e-9
 
 
 def test_f32k_download_b0_and_compensation_word(g):
