@@ -17,10 +17,9 @@
 //     into range, stores of lanes without an output dropped at offset
 //     2^31), so the compiler's wait before a step consumes its slot covers
 //     that slot's loads only;
-//   * workgroups: 512 threads (two per CU) for stage 0 of a standalone
-//     context, 1024 for stages 1-4 and the fields pass, 768 for the final
-//     stage; with pressure 768 for stages 0-4 and 512 for the final stage
-//     and the fields pass (the host picks the smallest that holds a rect);
+//   * workgroups: up to 1024 threads for stages 0-4 and the fields pass,
+//     768 for the final stage, 768 for every kernel with pressure (the host
+//     picks the smallest that holds a rect);
 //   * positions (hi, lo), B0 and the slopes are stored as (x, y, z)
 //     triples, one 12-byte load or store per point and array: a rect row
 //     of an array is one run of 12 (TX + 4) bytes, so its loads touch about
@@ -39,7 +38,10 @@
 //   * the final stage carries, per plane, sum_m (c_m - b5_m) k_m and
 //     sum_m (c_m - c*_m) k_m over the five stored slopes and adds
 //     c_5 k_5 at the point: x_new = x_5 + sum_m (c_m - b5_m) k_m by a
-//     two-sum into (hi, lo).
+//     two-sum into (hi, lo);
+//   * gradP from the cofactor vectors of the current, with the grid
+//     spacings cancelled, and its mixed second differences from a two-slot
+//     LDS ring of each point's y and z differences.
 #include "fused_common.hpp"
 #include "gm_device.hpp"
 #include "gm_fused.hpp"
@@ -154,27 +156,23 @@ __device__ __forceinline__ BufF buf_of(const float *p, long long nfloats)
     return BufF{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0,
                                                   (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL), 0x00020000)};
 }
-// Final stage: the per-plane sums (c - b5) k and (c - c*) k wait two plane
-// steps for the point's J.  With pressure they wait in LDS (a thread's own
-// two-slot ring), which takes the pressure variant from 185 VGPRs in
-// 512-thread workgroups to 168 in 768-thread ones (257 x 257 x 769: 2466
-// -> 1952 us); without pressure a three-plane register queue is as fast
-constexpr bool final_lds(bool press) { return press; }
-// The other pressure kernels put the y and z central differences of each
-// B-ring point in LDS for one plane: gradP's mixed second differences at a
-// tile point are differences of them between its in-plane neighbours,
+// The pressure kernels put the y and z central differences of each B-ring
+// point in LDS for one plane: gradP's mixed second differences at a tile
+// point are differences of them between its in-plane neighbours,
 // (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b)) = dr_b(+a) - dr_b(-a), the same
 // floating-point operations on the same values as from the positions, with
-// 8 LDS reads instead of 24
-constexpr bool dr_lds(int stage, bool press) { return press && stage != 5; }
+// 8 LDS reads instead of 24 (257 x 257 x 769 with pressure: stages 0-4 5%
+// faster, the final stage 7%).  The final stage's per-plane sums
+// (c - b5) k and (c - c*) k stay in registers: in LDS instead they cost the
+// final stage without pressure 5 us at 129 x 129 x 385, and take the LDS the
+// ring needs with pressure
+constexpr bool dr_lds(bool press) { return press; }
 template <int NT, int STAGE = 0, bool PRESS = false>
 constexpr size_t lds_bytes32()
 {
-    // positions, B, and one two-slot ring of a float4 and a float2 per
-    // thread for the final stage's sums or the pressure kernels' differences
+    // positions, B and, with pressure, the two-slot ring of differences
     return (size_t)kNXL * NT * (sizeof(float4) + sizeof(float2)) + (size_t)kNBL * NT * sizeof(float4) +
-           ((STAGE == 5 && final_lds(PRESS)) || dr_lds(STAGE, PRESS)
-                ? (size_t)2 * NT * (sizeof(float4) + sizeof(float2)) : 0);
+           (dr_lds(PRESS) ? (size_t)2 * NT * (sizeof(float4) + sizeof(float2)) : 0);
 }
 
 // det of an FP32 Jacobian in FP64 by cofactors (products of two floats are
@@ -230,16 +228,14 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     // the (runtime-off) branch: without it their stage 0 takes 30 more VGPRs
     constexpr bool EMIT = STAGE == 6 || PRESS;
     constexpr int kPF = pf_depth(STAGE, PRESS);
-    constexpr bool kFinalLds = FINAL && final_lds(PRESS);   // final-stage sums in LDS
-    constexpr bool kDrLds = dr_lds(STAGE, PRESS);           // y, z differences in LDS
+    constexpr bool kDrLds = dr_lds(PRESS);   // y, z differences in LDS
 
     extern __shared__ __align__(16) unsigned char smem[];
     Lds<NT> L;
     L.xa = reinterpret_cast<float4 *>(smem);
     L.xb = reinterpret_cast<float2 *>(L.xa + kNXL * NT);
     L.bl = reinterpret_cast<float4 *>(L.xb + kNXL * NT);
-    // two-slot ring [2][NT]: the final stage's sums (c - b5) k | (c - c*) k
-    // (kFinalLds), or dr_y | dr_z of a plane (kDrLds)
+    // two-slot ring [2][NT] of dr_y | dr_z of a plane (kDrLds)
     float4 *fqa = L.bl + kNBL * NT;
     float2 *fqb = reinterpret_cast<float2 *>(fqa + 2 * NT);
     (void)fqa; (void)fqb;
@@ -585,10 +581,10 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 #pragma unroll
                     for (int l = 0; l < 3; ++l) dd[a2][l] = dif(xp, xc, l) - dif(xc, xm, l);
                 }
-                // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b))
-                if constexpr (kDrLds) {
-                    // = dr_b(+a) - dr_b(-a) from the ring; dr_z carries the
-                    // march sign zs already (exact: zs = -+1)
+                // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b)) = dr_b(+a) -
+                // dr_b(-a), from the ring; dr_z carries the march sign zs
+                // already (exact: zs = -+1)
+                {
                     const int sl = (pj & 1) * NT;
                     const float4 ea = fqa[sl + t + 1], wa = fqa[sl + t - 1], na = fqa[sl + t + RW], sa = fqa[sl + t - RW];
                     const float2 eb = fqb[sl + t + 1], wb = fqb[sl + t - 1], nb = fqb[sl + t + RW], sb = fqb[sl + t - RW];
@@ -601,23 +597,6 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                         mxz[l] = dzE[l] - dzW[l];
                         myz[l] = dzN[l] - dzS[l];
                     }
-                } else {
-                    Pt3 pa, pb, pc, pd;
-                    // xy
-                    L.ldx(pj, t + 1 + RW, pa); L.ldx(pj, t + 1 - RW, pb);
-                    L.ldx(pj, t - 1 + RW, pc); L.ldx(pj, t - 1 - RW, pd);
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) mxy[l] = dif(pa, pb, l) - dif(pc, pd, l);
-                    // xz
-                    L.ldx(pj + 1, t + 1, pa); L.ldx(pj - 1, t + 1, pb);
-                    L.ldx(pj + 1, t - 1, pc); L.ldx(pj - 1, t - 1, pd);
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) mxz[l] = zs * (dif(pa, pb, l) - dif(pc, pd, l));
-                    // yz
-                    L.ldx(pj + 1, t + RW, pa); L.ldx(pj - 1, t + RW, pb);
-                    L.ldx(pj + 1, t - RW, pc); L.ldx(pj - 1, t - RW, pd);
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) myz[l] = zs * (dif(pa, pb, l) - dif(pc, pd, l));
                 }
                 auto dot = [](const float u[3], const float v[3]) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
                 const float N0 = 4.f * dot(dd[0], C[0]) + dot(mxy, C[1]) + dot(mxz, C[2]);
@@ -671,16 +650,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 const int va = 12 * (int)a;
                 Pt3 xc;
                 L.ldx(pj, t, xc);
-                float fg0[3], fe0[3];   // the sums of plane pj
-                if constexpr (kFinalLds) {
-                    const float4 qa = fqa[(pj & 1) * NT + t];
-                    const float2 qb = fqb[(pj & 1) * NT + t];
-                    fg0[0] = qa.x; fg0[1] = qa.y; fg0[2] = qa.z;
-                    fe0[0] = qa.w; fe0[1] = qb.x; fe0[2] = qb.y;
-                } else {
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) { fg0[l] = fg[0][l]; fe0[l] = fe[0][l]; }
-                }
+                const float (&fg0)[3] = fg[0], (&fe0)[3] = fe[0];   // the sums of plane pj
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
                     float hi = xc.h[l], lo;
@@ -708,12 +678,6 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         } else {
             bXhd.st3(hn, st_off);
             bXld.st3(ln, st_off);
-            // the sums of plane p+1, formed at the top of this step, into the
-            // slot plane p-1's were just read from
-            if (kFinalLds && ldc && plane_ok(p + 1)) {
-                fqa[((p + 1) & 1) * NT + t] = make_float4(fg[2][0], fg[2][1], fg[2][2], fe[2][0]);
-                fqb[((p + 1) & 1) * NT + t] = make_float2(fe[2][1], fe[2][2]);
-            }
         }
 
         // one barrier per plane
@@ -726,7 +690,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
             for (int l = 0; l < 3; ++l) drq[0][a2][l] = drq[1][a2][l];
-        if constexpr (FINAL && !final_lds(PRESS)) {
+        if constexpr (FINAL) {
 #pragma unroll
             for (int l = 0; l < 3; ++l) {
                 fg[0][l] = fg[1][l]; fg[1][l] = fg[2][l];
@@ -868,10 +832,10 @@ void launch_1024(hipStream_t s, const FusedArgs &A)
 int threads32(int rect) { return rect <= 256 ? 256 : rect <= 384 ? 384 : rect <= 512 ? 512 : rect <= 768 ? 768 : 1024; }
 
 // 768-thread workgroups (12 waves, 168 VGPRs, three waves per SIMD): every
-// stage kernel fits that budget without spilling (the pressure final stage
-// with its per-plane sums in LDS).  The final stage without pressure: 266
-// instead of 272 us at 129 x 129 x 385; the pressure stages 0-4 at
-// 257 x 257 x 769: 4-11% faster than in 512-thread workgroups
+// stage kernel fits that budget without spilling.  The final stage without
+// pressure: 266 instead of 272 us at 129 x 129 x 385 when it was set; the
+// pressure stages 0-4 at 257 x 257 x 769: 4-11% faster than in 512-thread
+// workgroups, the pressure final stage 2466 -> 1952 us
 constexpr bool fits768(int, bool) { return true; }
 
 int max_rect32(int stage, int press)
