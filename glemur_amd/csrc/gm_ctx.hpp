@@ -108,6 +108,7 @@ struct gm_ctx {
     // memory-side cache; on up to 16 M padded points (a stage moves up to
     // about 1.3 GB there), off for larger grids and ensemble members
     bool zrev = false;
+    bool zalt = false;
     // tiles
     // tiles per stage kernel (index 0..5 the stages, 6 the fields-only
     // pass): the stages exchange data through HBM only, so each takes the

@@ -31,6 +31,7 @@ struct FusedArgs {
     int face_table;                   // 1: every ghost face from the precomputed constants
                                       // (initial boundary distortion); 0: x/y faces direct (F32K)
     int rev;                          // F32K: march z downwards (plane q stored at nz + 1 - q)
+    int rev_alt;                      // F32K: odd z chunks march the other way
     // F64 buffers
     double *X64[3];                   // positions: current, candidate, stage-6 (3P each)
     const double *B0_64;

@@ -216,7 +216,7 @@ constexpr bool fits1024(int stage, bool press) { return !press && stage != 5; }
 // One workgroup's march: `lb` is its logical index in the instance (after
 // the XCD-aware remap), which picks the tile and z chunk and the slot of
 // its partial results.
-template <int NT, int STAGE, bool PRESS>
+template <int NT, int STAGE, bool PRESS, bool ALT>
 __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 {
 #pragma clang fp contract(fast)
@@ -251,11 +251,26 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const int Lb = lb;
     const int tx = Lb % A.tiles_x, ty = (Lb / A.tiles_x) % A.tiles_y, ch = Lb / (A.tiles_x * A.tiles_y);
     const int ix0 = tx * A.TX, iy0 = ty * A.TY;
-    const int Ks = ch * A.CZ + 1, Ke = min(ch * A.CZ + A.CZ, g.nz);
-    // the march runs over virtual planes q; with A.rev it goes down the grid:
-    // q lives at physical plane nz + 1 - q, and z differences of the march
-    // order change sign (zs)
-    const bool rev = A.rev != 0;
+    // the march runs over virtual planes q; downwards (A.rev), q lives at
+    // physical plane nz + 1 - q, and z differences of the march order change
+    // sign (zs).  With ALT odd chunks march the other way than A.rev says:
+    // neighbouring chunks then read the planes they share (two beyond each
+    // end) at about the same time, both first or both last, so the second
+    // read finds them in cache.  (A direction that depends on the workgroup
+    // changes the compiler's schedule of the whole kernel, by up to 5% on
+    // the pressure kernels, so it is a variant of its own.)
+    int Ks, Ke;
+    bool rev;
+    if constexpr (ALT) {
+        const int P0 = ch * A.CZ + 1, P1 = min(ch * A.CZ + A.CZ, g.nz);   // physical planes of the chunk
+        rev = (A.rev != 0) != ((ch & 1) != 0);
+        Ks = rev ? g.nz + 1 - P1 : P0;
+        Ke = rev ? g.nz + 1 - P0 : P1;
+    } else {
+        Ks = ch * A.CZ + 1;
+        Ke = min(ch * A.CZ + A.CZ, g.nz);
+        rev = A.rev != 0;
+    }
     auto zp = [&](int q) { return rev ? g.nz + 1 - q : q; };
     const float zs = rev ? -1.f : 1.f;
     const int RW = A.TX + 4, RH = A.TY + 4;
@@ -760,11 +775,11 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     }
 }
 
-template <int NT, int STAGE, bool PRESS>
+template <int NT, int STAGE, bool PRESS, bool ALT = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT, STAGE, PRESS), 8)))
 k_stage32(FusedArgs A)
 {
-    stage32_body<NT, STAGE, PRESS>(A, xcd_remap(blockIdx.x, A.nblocks));
+    stage32_body<NT, STAGE, PRESS, ALT>(A, xcd_remap(blockIdx.x, A.nblocks));
 }
 
 // Ensemble: one launch covers the workgroups of n instances; start[i] is
@@ -780,21 +795,32 @@ k_stage32_batch(const FusedArgs *__restrict__ args, const int *__restrict__ star
         if (start[mid] <= L) lo = mid;
         else hi = mid - 1;
     }
-    stage32_body<NT, STAGE, PRESS>(args[lo], L - start[lo]);
+    stage32_body<NT, STAGE, PRESS, false>(args[lo], L - start[lo]);
 }
 
 namespace {
-template <int NT, int STAGE, bool PRESS>
-void launch_one32(hipStream_t s, const FusedArgs &A)
+// chunk-alternating variants exist for stages 1-5 in 768- and 1024-thread
+// workgroups; other launches ignore A.rev_alt
+constexpr bool alt_built(int nt, int stage) { return nt >= 768 && stage >= 1 && stage <= 5; }
+template <int NT, int STAGE, bool PRESS, bool ALT>
+void launch_k32(hipStream_t s, const FusedArgs &A)
 {
     static bool attr = false;
     constexpr size_t lds = lds_bytes32<NT, STAGE, PRESS>();
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_stage32<NT, STAGE, PRESS>,
+        (void)hipFuncSetAttribute((const void *)k_stage32<NT, STAGE, PRESS, ALT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage32<NT, STAGE, PRESS>), dim3(A.nblocks), dim3(NT), lds, s, A);
+    hipLaunchKernelGGL((k_stage32<NT, STAGE, PRESS, ALT>), dim3(A.nblocks), dim3(NT), lds, s, A);
+}
+template <int NT, int STAGE, bool PRESS>
+void launch_one32(hipStream_t s, const FusedArgs &A)
+{
+    if constexpr (alt_built(NT, STAGE)) {
+        if (A.rev_alt) return launch_k32<NT, STAGE, PRESS, true>(s, A);
+    }
+    launch_k32<NT, STAGE, PRESS, false>(s, A);
 }
 template <int NT, bool PRESS>
 void launch_stage32(hipStream_t s, const FusedArgs &A)

@@ -217,6 +217,10 @@ gmd::FusedArgs fused_args(gm_ctx *c, int stage, int emit, int diag, int check_do
     // the ghost layer itself differ: those runs take every face from the table
     A.face_table = distorted_ghosts(c->p);
     A.rev = c->zrev && stage >= 0 && (stage & 1);
+    // odd chunks the other way when all of the launch's workgroups are
+    // resident at once (neighbouring chunks then run side by side); not
+    // stage 0, whose energy sum follows the march order
+    A.rev_alt = c->zalt && stage >= 1 && c->nblocks[k] <= c->ncu * stage_blocks_per_cu(c, k);
     for (int i = 0; i < 3; ++i) A.X64[i] = c->X64[i];
     A.B0_64 = c->B0_64; A.K64 = c->K64;
     A.eB64 = c->B64; A.eJ64 = c->J64; A.eG64 = c->G64;
@@ -323,6 +327,7 @@ extern "C" int gm_create(const gm_params *pp, int precision, gm_ctx **out)
     c->geo = make_geo(c->p);
     c->P = c->geo.P;
     c->zrev = precision == GM_PREC_F32K && c->P <= (16LL << 20);
+    c->zalt = precision == GM_PREC_F32K;
     c->N = (long long)pp->nx * pp->ny * pp->nz;
     const size_t P3 = 3 * (size_t)c->P;
     int r = GM_OK;
@@ -1483,6 +1488,7 @@ extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm
         if ((r = gm_create(&params[i], precision, &c))) return bail(r);
         b->inst.push_back(c);
         c->zrev = false;   // the members together move several GB per stage
+        c->zalt = false;
         auto_tiles(c, std::max(1, (c->ncu + n - 1) / n), true);
         b->ctlh.push_back(c->ctlh);
     }
