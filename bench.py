@@ -106,10 +106,19 @@ def threads32(rect):
     return 256 if rect <= 256 else 384 if rect <= 384 else 512 if rect <= 512 else 768 if rect <= 768 else 1024
 
 
+MI355X_CUS = 256
+
+
 def kernel_name(stage_tiles, press=False):
+    """The stage kernels a context launches with these tiles; the fourth
+    template argument is the variant whose odd z chunks march the other
+    way, taken by stages 1-5 in 768- or 1024-thread workgroups when the
+    launch's workgroups are all resident at once (one per CU)."""
     parts = []
     for st, (tx, ty, cz, nb) in enumerate(stage_tiles):
-        parts.append(f"stage {st}: k_stage32<{threads32((tx + 4) * (ty + 4))},{st},{str(press).lower()}> "
+        nt = threads32((tx + 4) * (ty + 4))
+        alt = 1 <= st <= 5 and nt >= 768 and nb <= MI355X_CUS
+        parts.append(f"stage {st}: k_stage32<{nt},{st},{str(press).lower()},{str(alt).lower()}> "
                      f"{tx}x{ty} tiles, {cz}-plane chunks, {nb} workgroups")
     return "; ".join(parts)
 
