@@ -79,6 +79,8 @@ int threads32(int rect);
 int blocks_per_cu32(int rect, int stage, int press);
 // Largest rect the FP32 kernel of this stage is built for (512, 768 or 1024).
 int max_rect32(int stage, int press);
+// Periodic ghost layers of the current F32K positions (buffer ctl->cur).
+void launch_set_grid_peri32(hipStream_t s, const Geo &g, float *const Xh[2], float *const Xl[2], const Ctl *ctl);
 void launch_controller(hipStream_t s, const CtlArgs &A);
 // Ensemble launches: every workgroup of n instances in one launch of the
 // FP32 stage kernel in `nt`-thread workgroups (512, or 768 / 1024 where the

@@ -216,7 +216,7 @@ constexpr bool fits1024(int stage, bool press) { return !press && stage != 5; }
 // One workgroup's march: `lb` is its logical index in the instance (after
 // the XCD-aware remap), which picks the tile and z chunk and the slot of
 // its partial results.
-template <int NT, int STAGE, bool PRESS, bool ALT>
+template <int NT, int STAGE, bool PRESS, bool ALT, bool PERI>
 __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 {
 #pragma clang fp contract(fast)
@@ -276,18 +276,29 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const int RW = A.TX + 4, RH = A.TY + 4;
     const int ixe = min(ix0 + A.TX, g.nx), iye = min(iy0 + A.TY, g.ny);
 
+    // periodic axes (PERI variants): columns, rows and planes beyond the
+    // inner range hold the opposite side's points shifted by -+(L + h) along
+    // the axis, the reference's ghost copies (src/boundaries.cu:532-582)
+    // formed at every stage from the opposite side's inputs; the ghost
+    // layer's B then takes the interior formula, which equals the
+    // reference's copy of the opposite layer (setPeri) in exact arithmetic
+    const bool px = PERI && g.xPeri, py = PERI && g.yPeri, pz = PERI && g.zPeri;
+
     // ---- this thread's column
     const int t = threadIdx.x;
     const int c = t % RW, r = t / RW;
     const int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-    const bool col = t < RW * RH && gi >= 0 && gi <= g.nx + 1 && gj >= 0 && gj <= g.ny + 1;
+    const bool col = PERI ? t < RW * RH && gi >= (px ? -1 : 0) && gi <= g.nx + (px ? 2 : 1) && gj >= (py ? -1 : 0) &&
+                                gj <= g.ny + (py ? 2 : 1)
+                          : t < RW * RH && gi >= 0 && gi <= g.nx + 1 && gj >= 0 && gj <= g.ny + 1;
     const bool gx = gi == 0 || gi == g.nx + 1, gy = gj == 0 || gj == g.ny + 1;
+    const bool lx = gx && !px, ly = gy && !py;   // ghost columns / rows of line-tied faces
     // virtual columns one step beyond a line-tied x or y face hold the outer
     // ghost x_g -+ h n of the face point next to them (src/boundaries.cu:
     // 164-186, 291-313), so the face point's Jacobian is the interior
     // central-difference formula and no wave runs a separate face branch
-    const int vx = t < RW * RH && gj >= 0 && gj <= g.ny + 1 ? (gi == -1 ? -1 : gi == g.nx + 2 ? 1 : 0) : 0;
-    const int vy = t < RW * RH && gi >= 0 && gi <= g.nx + 1 ? (gj == -1 ? -1 : gj == g.ny + 2 ? 1 : 0) : 0;
+    const int vx = (!PERI || !px) && t < RW * RH && gj >= 0 && gj <= g.ny + 1 ? (gi == -1 ? -1 : gi == g.nx + 2 ? 1 : 0) : 0;
+    const int vy = (!PERI || !py) && t < RW * RH && gi >= 0 && gi <= g.nx + 1 ? (gj == -1 ? -1 : gj == g.ny + 2 ? 1 : 0) : 0;
     const bool vcol = !A.face_table && (vx != 0 || vy != 0);
     const bool ldc = col || vcol;       // columns whose stage positions go to LDS
     const bool inB = col && c >= 1 && c <= RW - 2 && r >= 1 && r <= RH - 2 &&
@@ -301,7 +312,15 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     // counts them and waits only for the slot a step consumes)
     const int tl = min(t, RW * RH - 1);
     const int gil = ix0 - 1 + tl % RW, gjl = iy0 - 1 + tl / RW;
+    // abase: the lane's own padded point (stores); aload: the point it
+    // loads, on a periodic axis the opposite side's, shifted by sgx, sgy
+    // periods when its stage position is formed
     const long long abase = (long long)min(max(gil, 0), g.nx + 1) + g.sx * (long long)min(max(gjl, 0), g.ny + 1);
+    const int sgx = px ? (gil <= 0 ? -1 : gil >= g.nx + 1 ? 1 : 0) : 0;
+    const int sgy = py ? (gjl <= 0 ? -1 : gjl >= g.ny + 1 ? 1 : 0) : 0;
+    const long long aload = PERI ? (long long)(px ? gil - sgx * g.nx : min(max(gil, 0), g.nx + 1)) +
+                                       g.sx * (long long)(py ? gjl - sgy * g.ny : min(max(gjl, 0), g.ny + 1))
+                                 : abase;
     const int ii = gi - 1, jj = gj - 1;
     const float eps_xy = (g.epsilonProf && inJ) ? (float)(A.epsx[ii] * A.epsy[jj]) : 1.0f;
     // buffer views: positions (hi, lo), B0, one per slope slot; each holds
@@ -327,15 +346,23 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     float jmax2 = 0.f, r1q = 0.f, r2q = 0.f, r3q = 0.f, r4q = 0.f;   // stage-0 maxima, squared
     int nan_seen = 0;
 
-    auto plane_ok = [&](int q) { return q >= max(Ks - 2, 0) && q <= min(Ke + 2, g.nz + 1); };
+    // planes a chunk touches: two beyond its ends, within the padded grid
+    // (one more on each side of a periodic z axis)
+    auto qlo = [&]() { return PERI ? max(Ks - 2, pz ? -1 : 0) : max(Ks - 2, 0); };
+    auto qhi = [&]() { return PERI ? min(Ke + 2, g.nz + (pz ? 2 : 1)) : min(Ke + 2, g.nz + 1); };
+    auto plane_ok = [&](int q) { return q >= qlo() && q <= qhi(); };
+    // periodic z: the physical plane a virtual plane loads, and its period shift
+    auto zsrc = [&](int q) { const int z = zp(q); return pz ? (z <= 0 ? z + g.nz : z >= g.nz + 1 ? z - g.nz : z) : z; };
+    auto zshift = [&](int q) { const int z = zp(q); return pz ? (z <= 0 ? -1 : z >= g.nz + 1 ? 1 : 0) : 0; };
+    (void)zshift;
 
     // Every lane loads every step; planes outside the chunk's range repeat
     // its first or last plane (cache hits, no extra HBM traffic).  Slopes of
     // ghost points are zero in memory (never written), so a ghost point's
     // stage position needs no test
     auto load_plane = [&](int q, In &in) {
-        const int qc = min(max(q, max(Ks - 2, 0)), min(Ke + 2, g.nz + 1));
-        const int va = 4 * (int)(abase + g.sxy * (long long)zp(qc));
+        const int qc = min(max(q, qlo()), qhi());
+        const int va = PERI ? 4 * (int)(aload + g.sxy * (long long)zsrc(qc)) : 4 * (int)(abase + g.sxy * (long long)zp(qc));
         bXh.ld3(3 * va, in.x);
         bXl.ld3(3 * va, in.lo);
         bB0.ld3(3 * va, in.b0);
@@ -380,6 +407,24 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 if (vx) xn.s[0] += (float)vx * (float)g.dx;
                 if (vy) xn.s[1] += (float)vy * (float)g.dy;
             }
+            if constexpr (PERI) {
+                // periodic images: x -+ (L + h) on the axis, the period split
+                // into two floats and added by an exact two-sum to the high
+                // word, its rounding error and the period's low part to the
+                // small word
+                const int sg[3] = {sgx, sgy, zshift(p + 1)};
+                const double per[3] = {g.Lx + g.dx, g.Ly + g.dy, g.Lz + g.dz};
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    if (sg[l]) {
+                        const float ph = (float)per[l], pl = (float)(per[l] - (double)ph);
+                        float hs, e;
+                        two_sum(xn.h[l], (float)sg[l] * ph, hs, e);
+                        xn.h[l] = hs;
+                        xn.s[l] += e + (float)sg[l] * pl;
+                    }
+                }
+            }
             L.stx(p + 1, t, xn);
         }
         // (0) loads of plane p+1+kPF into the slot just read, in flight
@@ -392,15 +437,16 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1 &&
             !((gx || gy) && (p == 0 || p == g.nz + 1))) {
             const bool zg = p == 0 || p == g.nz + 1;
+            const bool zgl = zg && !pz;   // ghost plane of a line-tied z face
             float Bv[3];
             double detv;
             Pt3 xc, xzp, xzm;
-            if (!zg) {
+            if (!zgl) {
                 L.ldx(p, t, xc);
                 xzp = xn;                 // own plane p+1, formed above
                 L.ldx(p - 1, t, xzm);
             }
-            if (!zg && !(A.face_table && (gx || gy))) {
+            if (!zgl && !(A.face_table && (lx || ly))) {
                 // central differences of x along each axis, one neighbour
                 // pair at a time; they give the Jacobian here and, kept for
                 // one plane, the current at this point in the next iteration.
@@ -436,14 +482,14 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 #pragma unroll
                 for (int l = 0; l < 3; ++l)
                     Bv[l] = (drq[1][0][l] * bs[0] + drq[1][1][l] * bs[1] + drq[1][2][l] * bs[2]) * rd;
-                if (gx) Bv[0] = b0q[0][0];
-                if (gy) Bv[1] = b0q[0][1];
+                if (lx) Bv[0] = b0q[0][0];
+                if (ly) Bv[1] = b0q[0][1];
             } else {
                 // z faces (and every face of a grid with an initial boundary
                 // distortion): precomputed face constants
                 int face, side, u, v;
                 double xin[3];
-                if (zg) {
+                if (zgl) {
                     face = 2;
                     side = zp(p) != 0;
                     u = ii;
@@ -454,9 +500,9 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 #pragma unroll
                     for (int l = 0; l < 3; ++l) xin[l] = (double)pin.h[l] + (double)pin.s[l];
                 } else {
-                    face = gx ? 0 : 1;
-                    side = gx ? (gi != 0) : (gj != 0);
-                    u = gx ? jj : ii;
+                    face = lx ? 0 : 1;
+                    side = lx ? (gi != 0) : (gj != 0);
+                    u = lx ? jj : ii;
                     v = zp(p) - 1;
                     const int st = face == 0 ? 1 : RW;
                     Pt3 pin;
@@ -472,7 +518,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 if constexpr (kDrLds) {
                     // x and y faces from the table: their y and z differences
                     // still feed the mixed terms of the tile points beside them
-                    if (!zg) {
+                    if (!zgl) {
                         Pt3 a, b;
                         L.ldx(p, t + RW, a);
                         L.ldx(p, t - RW, b);
@@ -485,7 +531,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 }
             }
             if constexpr (kDrLds) {
-                if (!zg) {
+                if (!zgl) {
                     fqa[(p & 1) * NT + t] = make_float4(drq[1][1][0], drq[1][1][1], drq[1][1][2], drq[1][2][0]);
                     fqb[(p & 1) * NT + t] = make_float2(drq[1][2][1], drq[1][2][2]);
                 }
@@ -775,11 +821,11 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     }
 }
 
-template <int NT, int STAGE, bool PRESS, bool ALT = false>
+template <int NT, int STAGE, bool PRESS, bool ALT = false, bool PERI = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves32(NT, STAGE, PRESS), 8)))
 k_stage32(FusedArgs A)
 {
-    stage32_body<NT, STAGE, PRESS, ALT>(A, xcd_remap(blockIdx.x, A.nblocks));
+    stage32_body<NT, STAGE, PRESS, ALT, PERI>(A, xcd_remap(blockIdx.x, A.nblocks));
 }
 
 // Ensemble: one launch covers the workgroups of n instances; start[i] is
@@ -795,24 +841,24 @@ k_stage32_batch(const FusedArgs *__restrict__ args, const int *__restrict__ star
         if (start[mid] <= L) lo = mid;
         else hi = mid - 1;
     }
-    stage32_body<NT, STAGE, PRESS, false>(args[lo], L - start[lo]);
+    stage32_body<NT, STAGE, PRESS, false, false>(args[lo], L - start[lo]);
 }
 
 namespace {
 // chunk-alternating variants exist for stages 1-5 in 768- and 1024-thread
 // workgroups; other launches ignore A.rev_alt
 constexpr bool alt_built(int nt, int stage) { return nt >= 768 && stage >= 1 && stage <= 5; }
-template <int NT, int STAGE, bool PRESS, bool ALT>
+template <int NT, int STAGE, bool PRESS, bool ALT, bool PERI = false>
 void launch_k32(hipStream_t s, const FusedArgs &A)
 {
     static bool attr = false;
     constexpr size_t lds = lds_bytes32<NT, STAGE, PRESS>();
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_stage32<NT, STAGE, PRESS, ALT>,
+        (void)hipFuncSetAttribute((const void *)k_stage32<NT, STAGE, PRESS, ALT, PERI>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage32<NT, STAGE, PRESS, ALT>), dim3(A.nblocks), dim3(NT), lds, s, A);
+    hipLaunchKernelGGL((k_stage32<NT, STAGE, PRESS, ALT, PERI>), dim3(A.nblocks), dim3(NT), lds, s, A);
 }
 template <int NT, int STAGE, bool PRESS>
 void launch_one32(hipStream_t s, const FusedArgs &A)
@@ -840,6 +886,21 @@ void launch_nt32(hipStream_t s, const FusedArgs &A)
 {
     if (A.g.pressure) launch_stage32<NT, true>(s, A);
     else launch_stage32<NT, false>(s, A);
+}
+// grids with periodic faces: 512-thread workgroups for every stage (the
+// host keeps their rects within 512 points)
+template <bool PRESS>
+void launch_peri32(hipStream_t s, const FusedArgs &A)
+{
+    switch (A.stage) {
+    case 0: launch_k32<512, 0, PRESS, false, true>(s, A); break;
+    case 1: launch_k32<512, 1, PRESS, false, true>(s, A); break;
+    case 2: launch_k32<512, 2, PRESS, false, true>(s, A); break;
+    case 3: launch_k32<512, 3, PRESS, false, true>(s, A); break;
+    case 4: launch_k32<512, 4, PRESS, false, true>(s, A); break;
+    case 5: launch_k32<512, 5, PRESS, false, true>(s, A); break;
+    default: launch_k32<512, 6, PRESS, false, true>(s, A); break;
+    }
 }
 // 1024 threads: stages 0-4 and the fields pass without pressure
 void launch_1024(hipStream_t s, const FusedArgs &A)
@@ -878,6 +939,63 @@ int blocks_per_cu32(int rect, int stage, int press)
 }
 
 void set_tableau32(const TableauF &t) { (void)hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), &t, sizeof t); }
+
+// Periodic ghost layers of the current F32K positions (setGridPeriFace,
+// src/boundaries.cu:532-582): the ghost layer takes the opposite inner
+// layer, shifted by -+(L + h) along the axis, with (hi + lo) -+ (L + h)
+// formed in FP64 and split again.  The stage kernels form these images
+// themselves; the layers serve downloads and diagnostics.
+__global__ void k_set_grid_peri32(Geo g, float *Xh0, float *Xl0, float *Xh1, float *Xl1, const Ctl *ctl, int face)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int na = face == 0 ? g.ny + 2 : g.nx + 2, nb = face == 2 ? g.ny + 2 : g.nz + 2;
+    if (t >= (long long)na * nb) return;
+    const int cur = ctl->cur;
+    float *Xh = cur ? Xh1 : Xh0, *Xl = cur ? Xl1 : Xl0;
+    const int u = (int)(t % na), v = (int)(t / na);
+    long long lo, hi, srclo, srchi;
+    double per;
+    if (face == 2) {
+        lo = gidx(g, u, v, 0); hi = gidx(g, u, v, g.nz + 1);
+        srclo = gidx(g, u, v, g.nz); srchi = gidx(g, u, v, 1);
+        per = g.Lz + g.dz;
+    } else if (face == 1) {
+        lo = gidx(g, u, 0, v); hi = gidx(g, u, g.ny + 1, v);
+        srclo = gidx(g, u, g.ny, v); srchi = gidx(g, u, 1, v);
+        per = g.Ly + g.dy;
+    } else {
+        lo = gidx(g, 0, u, v); hi = gidx(g, g.nx + 1, u, v);
+        srclo = gidx(g, g.nx, u, v); srchi = gidx(g, 1, u, v);
+        per = g.Lx + g.dx;
+    }
+    for (int l = 0; l < 3; ++l) {
+        double vlo = (double)Xh[3 * srclo + l] + (double)Xl[3 * srclo + l];
+        double vhi = (double)Xh[3 * srchi + l] + (double)Xl[3 * srchi + l];
+        if (l == face) {
+            vlo -= per;
+            vhi += per;
+        }
+        const float hlo = (float)vlo, hhi = (float)vhi;
+        Xh[3 * lo + l] = hlo;
+        Xl[3 * lo + l] = (float)(vlo - (double)hlo);
+        Xh[3 * hi + l] = hhi;
+        Xl[3 * hi + l] = (float)(vhi - (double)hhi);
+    }
+}
+
+void launch_set_grid_peri32(hipStream_t s, const Geo &g, float *const Xh[2], float *const Xl[2], const Ctl *ctl)
+{
+    // z, then y, then x, as the reference: edges and corners take the
+    // composed images
+    const int face_on[3] = {g.xPeri, g.yPeri, g.zPeri};
+    for (int face = 2; face >= 0; --face) {
+        if (!face_on[face]) continue;
+        const long long n = face == 2 ? (long long)(g.nx + 2) * (g.ny + 2)
+                          : face == 1 ? (long long)(g.nx + 2) * (g.nz + 2) : (long long)(g.ny + 2) * (g.nz + 2);
+        hipLaunchKernelGGL(k_set_grid_peri32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, Xh[0], Xl[0],
+                           Xh[1], Xl[1], ctl, face);
+    }
+}
 
 namespace {
 template <int NT, int STAGE, bool PRESS>
@@ -929,7 +1047,13 @@ void launch_fused32_batch(hipStream_t s, int stage, int press, int nt, const Fus
 
 void launch_fused32(hipStream_t s, const FusedArgs &A)
 {
-    // the smallest workgroup that holds the rect (the caller checked <= 512)
+    if (A.g.xPeri || A.g.yPeri || A.g.zPeri) {
+        if (A.g.pressure) launch_peri32<true>(s, A);
+        else launch_peri32<false>(s, A);
+        return;
+    }
+    // the smallest workgroup that holds the rect (the runtime checked the
+    // stage kernel's limit)
     switch (threads32((A.TX + 4) * (A.TY + 4))) {
     case 256: launch_nt32<256>(s, A); break;
     case 384: launch_nt32<384>(s, A); break;

@@ -94,16 +94,19 @@ bool distorted_ghosts(const gm_params &p)
 }
 
 // Workgroups per CU a stage kernel is built for (k: 0..5 stages, 6 fields).
+// (F32K grids with periodic faces run every stage in 512-thread workgroups.)
 int stage_blocks_per_cu(const gm_ctx *c, int k)
 {
     if (c->prec != GM_PREC_F32K) return 1;
-    return gmd::blocks_per_cu32((c->TX[k] + 4) * (c->TY[k] + 4), k == 6 ? -1 : k, c->p.pressure);
+    const int rect = any_peri(c->p) ? 512 : (c->TX[k] + 4) * (c->TY[k] + 4);
+    return gmd::blocks_per_cu32(rect, k == 6 ? -1 : k, c->p.pressure);
 }
 
 // Largest rect a stage kernel holds (one point per thread).
 int stage_max_rect(const gm_ctx *c, int k)
 {
     if (c->prec != GM_PREC_F32K) return gmd::fused_max_points(0);
+    if (any_peri(c->p)) return 512;
     return gmd::max_rect32(k == 6 ? -1 : k, c->p.pressure);
 }
 
@@ -265,6 +268,8 @@ void enqueue_step(gm_ctx *c, int diag, int check_done)
     ev_begin(c, 6);
     gmd::launch_controller(c->stream, C);
     ev_end(c);
+    // periodic ghost layers of the positions the controller left current
+    if (f32k && any_peri(c->p)) gmd::launch_set_grid_peri32(c->stream, c->geo, c->Xh, c->Xl, c->ctl);
 }
 
 int sync_ctl(gm_ctx *c)
@@ -477,6 +482,8 @@ extern "C" int gm_upload(gm_ctx *c, const double *B0, const double *xb, double t
     c->cur = 0;
     int r2 = push_ctl(c);
     if (r2) return r2;
+    // periodic ghost layers as the reference's setGridPeriHost leaves them
+    if (c->prec == GM_PREC_F32K && any_peri(c->p)) gmd::launch_set_grid_peri32(s, c->geo, c->Xh, c->Xl, c->ctl);
     // constants of the line-tied ghost faces (the ghost layer never moves)
     gmd::FusedArgs A = fused_args(c, 0, 0, 0, 0);
     gmd::launch_ghost_setup(c->stream, A, c->prec == GM_PREC_F32K, c->ghost);
@@ -838,8 +845,8 @@ extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
 // ------------------------------------------------------------ fused path
 static int check_fused(gm_ctx *c)
 {
-    if (any_peri(c->p))
-        return fail(GM_ERR_UNSUPPORTED, "periodic boundaries run through the per-kernel path (gm_b_jacb0 ...)");
+    if (any_peri(c->p) && c->prec != GM_PREC_F32K)
+        return fail(GM_ERR_UNSUPPORTED, "periodic boundaries of FP64 contexts run through the per-kernel path (gm_b_jacb0 ...)");
     if (!gm::starts(c->p.jMethod, "Stokes "))
         return fail(GM_ERR_UNSUPPORTED, "jMethod Classic / Stokes4th run through the per-kernel path (gm_current ...)");
     if (c->p.jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver runs through the per-kernel path (gm_rk_kk ...)");
