@@ -630,17 +630,28 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 // 45 operations on top of the differences instead of about 190
                 // for the inverse Jacobian and the 27-term sum
                 float dd[3][3], mxy[3], mxz[3], myz[3];
+                // diagonal: x(+a) - 2x + x(-a) = 2 (x(+a) - x) - dr_a along x and
+                // y, with the point's own central differences dr: one
+                // compensated difference per axis instead of two, and two
+                // position reads fewer (257 x 257 x 769 with pressure: about
+                // 3% faster).  Along z, whose march direction may reverse,
+                // (x(+) - x) - (x - x(-)), which a reversal maps to itself bit
+                // for bit (the form above would round differently)
                 Pt3 xc;
                 L.ldx(pj, t, xc);
-                const int vq[6] = {pj, pj, pj, pj, pj - 1, pj + 1};
-                const int vt[6] = {t - 1, t + 1, t - RW, t + RW, t, t};
 #pragma unroll
-                for (int a2 = 0; a2 < 3; ++a2) {
+                for (int a2 = 0; a2 < 2; ++a2) {
+                    Pt3 xp;
+                    L.ldx(pj, a2 == 0 ? t + 1 : t + RW, xp);
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) dd[a2][l] = fmaf(2.f, dif(xp, xc, l), -dr[a2][l]);
+                }
+                {
                     Pt3 xm, xp;
-                    L.ldx(vq[2 * a2], vt[2 * a2], xm);
-                    L.ldx(vq[2 * a2 + 1], vt[2 * a2 + 1], xp);
+                    L.ldx(pj - 1, t, xm);
+                    L.ldx(pj + 1, t, xp);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) dd[a2][l] = dif(xp, xc, l) - dif(xc, xm, l);
+                    for (int l = 0; l < 3; ++l) dd[2][l] = dif(xp, xc, l) - dif(xc, xm, l);
                 }
                 // mixed: (x(+a+b) - x(+a-b)) - (x(-a+b) - x(-a-b)) = dr_b(+a) -
                 // dr_b(-a), from the ring; dr_z carries the march sign zs
