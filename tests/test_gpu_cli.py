@@ -173,3 +173,37 @@ def test_batched_rows_from_step_log(require_gpu, tmp_path, prec):
     assert da == db and "save.vtk" in da and any(f.startswith("dump") for f in da)
     for f in da:
         assert (a / "data" / f).read_bytes() == (b / "data" / f).read_bytes(), f
+
+
+@pytest.mark.parametrize("faces", ["line-tied", "periodic"])
+def test_both_precisions_through_the_executable(require_gpu, tmp_path, faces):
+    """The executable with all faces line-tied and with xPeri = zPeri = t
+    (FP64 periodic runs step on the per-kernel path, FP32 ones, -p 32, on
+    the fused stage kernels).  Both precisions write every row.  From dt0 =
+    3e-4 the adaptive dt grows at a rate set by the error estimate, which
+    FP32 rounding dominates while dt is small, so the two runs take their own
+    step sequences and reach each row at their own t; compared at the same t
+    (linear interpolation between FP64 rows), their energies B2 differ by the
+    time discretisation of two different adaptive sequences (each step's
+    local error up to maxError), under 2e-4 relative here, alike for both
+    kinds of faces; the FP32 energy never rises."""
+    extra = ", xPeri = t, zPeri = t" if faces == "periodic" else ""
+    rows = {}
+    for prec in ("64", "32"):
+        d = tmp_path / prec
+        os.makedirs(d)
+        run_cli(d, "-p", prec, ncycle=30, run_extra=extra)
+        h, r = ts_rows(d)
+        off = 1 if h[0] == "#" else 0
+        rows[prec] = np.array([(float(x[h.index("t") - off]), float(x[h.index("B2") - off])) for x in r])
+        assert (d / "data" / "save.vtk").exists()
+    a, b = rows["64"], rows["32"]
+    assert len(a) == len(b) == 15
+    assert np.all(np.diff(b[:, 1]) <= 0)
+    inside = (b[:, 0] >= a[0, 0]) & (b[:, 0] <= a[-1, 0])
+    assert inside.sum() >= 5
+    want = np.interp(b[inside, 0], a[:, 0], a[:, 1])
+    err = np.max(np.abs(b[inside, 1] - want) / want)
+    print(f"{faces}: B2 of the FP32 run against the FP64 run at the same t: {err:.2e} ({inside.sum()} rows); "
+          f"t at the last row {a[-1, 0]:.4f} (FP64), {b[-1, 0]:.4f} (FP32)")
+    assert err < 2e-4

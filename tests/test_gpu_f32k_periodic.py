@@ -35,11 +35,15 @@ CASES = [
     {"xPeri": True, "yPeri": True, "zPeri": True},
     {"zPeri": True, "pressure": True, "beta": 0.05},
     {"xPeri": True, "yPeri": True, "pressure": True, "beta": 0.05, "epsilonProf": True},
+    # an initial boundary distortion: the line-tied faces take the face
+    # table, the periodic ones the wrapped images
+    {"xPeri": True, "pressure": True, "beta": 0.05, "initDist": b"initShearX", "initShearA": 0.05,
+     "initShearB": 0.02},
 ]
 
 
 def cid(case):
-    return "-".join(k for k, v in case.items() if v is True)
+    return "-".join([k for k, v in case.items() if v is True] + (["shear"] if "initDist" in case else []))
 
 
 @pytest.mark.parametrize("tile", [None, (6, 5, 7)], ids=["default_tile", "tiles_6x5x7"])
