@@ -41,7 +41,10 @@
 //     two-sum into (hi, lo);
 //   * gradP from the cofactor vectors of the current, with the grid
 //     spacings cancelled, and its mixed second differences from a two-slot
-//     LDS ring of each point's y and z differences.
+//     LDS ring of each point's y and z differences;
+//   * variants: odd z chunks marching the other way (stages 1-5, launches
+//     of one round), and periodic faces (512 threads), whose ghost images
+//     the kernel forms from the opposite side's inputs at every stage.
 #include "fused_common.hpp"
 #include "gm_device.hpp"
 #include "gm_fused.hpp"
