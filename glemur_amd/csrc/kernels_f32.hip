@@ -336,10 +336,13 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 #pragma unroll
     for (int m = 0; m < 6; ++m) bK[m] = buf_of(A.K32 + (long long)(m < 5 ? m : 4) * P3, P3);
 
-    // ---- registers carried between planes
+    // ---- registers carried between planes.  A new plane's differences and
+    // det are locals of its plane step, set only in the branch that uses
+    // them: carried through the branches that set them, the old plane's
+    // values would be copied between registers on every plane
     float b0q[2][3];      // B0 at planes p, p+1
-    double dq[2];         // det at planes p-1, p
-    float drq[2][3][3];   // central differences x(+a) - x(-a) at planes p-1, p
+    double dq0;           // det at plane p-1
+    float drq0[3][3];     // central differences x(+a) - x(-a) at plane p-1
     struct In {
         float x[3], lo[3], k[5][3], b0[3];
     } pf0, pf1;           // inputs of one plane, loaded kPF plane steps ahead
@@ -380,7 +383,26 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     // slots the loop is unrolled by two so that each step names its slot
     // statically (no register indexing)
     auto plane_step = [&](const int p, In &pf) {
-        // (1) stage position of plane p+1 from the inputs loaded kPF iterations ago
+        float drn[3][3];   // this plane's differences and det, set by (2)
+        double dqn;
+        // (1) stage position of plane p+1 from the inputs loaded kPF
+        // iterations ago.  Every lane loads every plane, so B0 and the final
+        // stage's sums of plane p+1 are taken without a branch (lanes and
+        // planes outside the range use none of them)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            b0q[1][l] = pf.b0[l];
+            if constexpr (FINAL) {
+                float a1 = 0, a2 = 0;
+#pragma unroll
+                for (int m = 0; m < 5; ++m) {
+                    a1 += (float)(c_tab32.c[m] - c_tab32.b[5 * 6 + m]) * pf.k[m][l];
+                    a2 += (float)c_tab32.ce[m] * pf.k[m][l];
+                }
+                fg[2][l] = a1;
+                fe[2][l] = a2;
+            }
+        }
         Pt3 xn;
         if (ldc && plane_ok(p + 1)) {
 #pragma unroll
@@ -391,17 +413,6 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     if (m < n) s += (float)c_tab32.b[n * 6 + m] * pf.k[m][l];
                 xn.h[l] = pf.x[l];
                 xn.s[l] = s;
-                b0q[1][l] = pf.b0[l];
-                if constexpr (FINAL) {
-                    float a1 = 0, a2 = 0;
-#pragma unroll
-                        for (int m = 0; m < 5; ++m) {
-                            a1 += (float)(c_tab32.c[m] - c_tab32.b[5 * 6 + m]) * pf.k[m][l];
-                            a2 += (float)c_tab32.ce[m] * pf.k[m][l];
-                        }
-                    fg[2][l] = a1;
-                    fe[2][l] = a2;
-                }
             }
             if (vcol) {
                 // outer ghost: the face point shifted by h along the face
@@ -463,7 +474,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     L.ldx(p, t + 1, a);
                     L.ldx(p, t - 1, b);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) drq[1][0][l] = dif(a, b, l);
+                    for (int l = 0; l < 3; ++l) drn[0][l] = dif(a, b, l);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 {
@@ -471,21 +482,21 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                     L.ldx(p, t + RW, a);
                     L.ldx(p, t - RW, b);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) drq[1][1][l] = dif(a, b, l);
+                    for (int l = 0; l < 3; ++l) drn[1][l] = dif(a, b, l);
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) drq[1][2][l] = zs * dif(xzp, xzm, l);
+                for (int l = 0; l < 3; ++l) drn[2][l] = zs * dif(xzp, xzm, l);
                 // jac = dr diag(hx, hy, hz): det(jac) = det(dr) hx hy hz
                 // (det(dr) from its rows, the transpose of jac's columns),
                 // and jac B0 = dr (h * B0), so the spacing scales the three
                 // components of B0 instead of the nine of dr
-                detv = det32(drq[1]) * ((double)hx * (double)hy * (double)hz);
+                detv = det32(drn) * ((double)hx * (double)hy * (double)hz);
                 const float rd = __builtin_amdgcn_rcpf((float)detv);
                 const float bs[3] = {b0q[0][0] * hx, b0q[0][1] * hy, b0q[0][2] * hz};
 #pragma unroll
                 for (int l = 0; l < 3; ++l)
-                    Bv[l] = (drq[1][0][l] * bs[0] + drq[1][1][l] * bs[1] + drq[1][2][l] * bs[2]) * rd;
+                    Bv[l] = (drn[0][l] * bs[0] + drn[1][l] * bs[1] + drn[2][l] * bs[2]) * rd;
                 if (lx) Bv[0] = b0q[0][0];
                 if (ly) Bv[1] = b0q[0][1];
             } else {
@@ -528,16 +539,16 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                         L.ldx(p, t - RW, b);
 #pragma unroll
                         for (int l = 0; l < 3; ++l) {
-                            drq[1][1][l] = dif(a, b, l);
-                            drq[1][2][l] = zs * dif(xzp, xzm, l);
+                            drn[1][l] = dif(a, b, l);
+                            drn[2][l] = zs * dif(xzp, xzm, l);
                         }
                     }
                 }
             }
             if constexpr (kDrLds) {
                 if (!zgl) {
-                    fqa[(p & 1) * NT + t] = make_float4(drq[1][1][0], drq[1][1][1], drq[1][1][2], drq[1][2][0]);
-                    fqb[(p & 1) * NT + t] = make_float2(drq[1][2][1], drq[1][2][2]);
+                    fqa[(p & 1) * NT + t] = make_float4(drn[1][0], drn[1][1], drn[1][2], drn[2][0]);
+                    fqb[(p & 1) * NT + t] = make_float2(drn[2][1], drn[2][2]);
                 }
             }
             if (EMIT && A.emit && (gx || gy || zg)) {
@@ -550,7 +561,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 }
             }
             L.stb(p, t, Bv);
-            dq[1] = detv;
+            dqn = detv;
         }
 
         __builtin_amdgcn_sched_barrier(0);
@@ -594,7 +605,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 #pragma unroll
                 for (int l = 0; l < 3; ++l) db[2][l] = zs * (bp[l] - bm[l]);
             }
-            const float (&dr)[3][3] = drq[0];
+            const float (&dr)[3][3] = drq0;
             // cofactor vectors C_a = dr[a+1] x dr[a+2]: the rows of
             // det(dr) dr^-1 (dr's columns are the three differences)
             float C[3][3];
@@ -609,7 +620,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
                 Jv[2] = fmaf(C[a0][0], db[a0][1], fmaf(-C[a0][1], db[a0][0], Jv[2]));
             }
             // det[dx dy dz] = 8 hx hy hz det(jac), det(jac) from FP64
-            const float rD = __builtin_amdgcn_rcpf((float)(dq[0] * (8.0 * g.dx * g.dy * g.dz)));
+            const float rD = __builtin_amdgcn_rcpf((float)(dq0 * (8.0 * g.dx * g.dy * g.dz)));
 #pragma unroll
             for (int l = 0; l < 3; ++l) Jv[l] *= rD;
             kv[0] = Jv[1] * Bc[2] - Jv[2] * Bc[1];
@@ -619,7 +630,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             float eps = 1.f;
             if (g.epsilonProf) eps = eps_xy * (float)A.epsz[kk];
             float gv[3] = {0, 0, 0};
-            const double detv = dq[0];
+            const double detv = dq0;
             if constexpr (PRESS) {
                 // gradP of src/gradP.cu:93-154,
                 //   g_c = -beta det1 sum_{a,b,l} d2[a][b][l] jac1[b][a] jac1[l][c]
@@ -761,11 +772,11 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
 
 #pragma unroll
         for (int l = 0; l < 3; ++l) b0q[0][l] = b0q[1][l];
-        dq[0] = dq[1];
+        dq0 = dqn;
 #pragma unroll
         for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
-            for (int l = 0; l < 3; ++l) drq[0][a2][l] = drq[1][a2][l];
+            for (int l = 0; l < 3; ++l) drq0[a2][l] = drn[a2][l];
         if constexpr (FINAL) {
 #pragma unroll
             for (int l = 0; l < 3; ++l) {
