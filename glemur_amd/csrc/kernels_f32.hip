@@ -131,10 +131,10 @@ struct BufF {
     // The loaded vector is bit-cast whole: a bit_cast of one of its
     // elements compiles to a single-dword load whose value lands in every
     // component
-    __device__ __forceinline__ void ld3(int voff, float o[3]) const
+    __device__ __forceinline__ void ld3(int voff, float o[3], int soff = 0) const
     {
         typedef float v3f __attribute__((ext_vector_type(3)));
-        const v3f f = __builtin_bit_cast(v3f, __builtin_amdgcn_raw_buffer_load_b96(r, voff, 0, 0));
+        const v3f f = __builtin_bit_cast(v3f, __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0));
         o[0] = f[0];
         o[1] = f[1];
         o[2] = f[2];
@@ -325,6 +325,11 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     const long long aload = PERI ? (long long)(px ? gil - sgx * g.nx : min(max(gil, 0), g.nx + 1)) +
                                        g.sx * (long long)(py ? gjl - sgy * g.ny : min(max(gjl, 0), g.ny + 1))
                                  : abase;
+    // byte offsets of 12-byte points: the lane's point within a plane (for
+    // its loads and its stores) and, per plane, the plane's first byte, a
+    // workgroup-uniform value that the loads take in their scalar offset
+    const int vl12 = 12 * (int)aload, vs12 = 12 * (int)abase;
+    const int pl12 = 12 * (int)g.sxy;
     const int ii = gi - 1, jj = gj - 1;
     const float eps_xy = (g.epsilonProf && inJ) ? (float)(A.epsx[ii] * A.epsy[jj]) : 1.0f;
     // buffer views: positions (hi, lo), B0, one per slope slot; each holds
@@ -369,13 +374,13 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
     // stage position needs no test
     auto load_plane = [&](int q, In &in) {
         const int qc = min(max(q, qlo()), qhi());
-        const int va = PERI ? 4 * (int)(aload + g.sxy * (long long)zsrc(qc)) : 4 * (int)(abase + g.sxy * (long long)zp(qc));
-        bXh.ld3(3 * va, in.x);
-        bXl.ld3(3 * va, in.lo);
-        bB0.ld3(3 * va, in.b0);
+        const int so = pl12 * (PERI ? zsrc(qc) : zp(qc));
+        bXh.ld3(vl12, in.x, so);
+        bXl.ld3(vl12, in.lo, so);
+        bB0.ld3(vl12, in.b0, so);
 #pragma unroll
         for (int m = 0; m < 5; ++m)
-            if (m < n) bK[m].ld3(3 * va, in.k[m]);   // xyz triples at 12 a
+            if (m < n) bK[m].ld3(vl12, in.k[m], so);   // xyz triples at 12 a
     };
 
     // ---- march.  Plane p+1+kPF is loaded at step p into one of kPF input
@@ -576,7 +581,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
         (void)kst; (void)hn; (void)ln;
         if (inJ && pj >= Ks && pj <= Ke) {
             const long long a = abase + g.sxy * (long long)zp(pj);
-            st_off = 12 * (int)a;
+            st_off = vs12 + pl12 * zp(pj);
             // Stokes current from central differences only.  For the
             // diamond through the face neighbours v0..v3 of the reference
             // (src/current.cu:291-382) the trapezoid circulation is
@@ -734,7 +739,7 @@ __device__ __forceinline__ void stage32_body(const FusedArgs &A, const int lb)
             } else {
                 // x_new = x_5 + sum_m (c_m - b5_m) k_m (k_5 included), into
                 // (hi, lo) by a two-sum; error |sum_m (c_m - c*_m) k_m| in FP64
-                const int va = 12 * (int)a;
+                const int va = st_off;   // the point's byte offset, 12 a
                 Pt3 xc;
                 L.ldx(pj, t, xc);
                 const float (&fg0)[3] = fg[0], (&fe0)[3] = fe[0];   // the sums of plane pj
