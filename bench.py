@@ -113,12 +113,13 @@ def kernel_name(stage_tiles, press=False):
     """The stage kernels a context launches with these tiles; the fourth
     template argument is the variant whose odd z chunks march the other
     way, taken by stages 1-5 in 768- or 1024-thread workgroups when the
-    launch's workgroups are all resident at once (one per CU)."""
+    launch's workgroups are all resident at once (one per CU), the fifth
+    the periodic-face variant (false: the benched braids are line-tied)."""
     parts = []
     for st, (tx, ty, cz, nb) in enumerate(stage_tiles):
         nt = threads32((tx + 4) * (ty + 4))
         alt = 1 <= st <= 5 and nt >= 768 and nb <= MI355X_CUS
-        parts.append(f"stage {st}: k_stage32<{nt},{st},{str(press).lower()},{str(alt).lower()}> "
+        parts.append(f"stage {st}: k_stage32<{nt},{st},{str(press).lower()},{str(alt).lower()},false> "
                      f"{tx}x{ty} tiles, {cz}-plane chunks, {nb} workgroups")
     return "; ".join(parts)
 
