@@ -81,6 +81,8 @@ int blocks_per_cu32(int rect, int stage, int press);
 int max_rect32(int stage, int press);
 // Periodic ghost layers of the current F32K positions (buffer ctl->cur).
 void launch_set_grid_peri32(hipStream_t s, const Geo &g, float *const Xh[2], float *const Xl[2], const Ctl *ctl);
+// FP64 periodic ghost layers of the positions in buffer ctl->cur (read on the device)
+void launch_set_grid_peri64(hipStream_t s, const FusedArgs &A);
 void launch_controller(hipStream_t s, const CtlArgs &A);
 // Ensemble launches: every workgroup of n instances in one launch of the
 // FP32 stage kernel in `nt`-thread workgroups (512, or 768 / 1024 where the

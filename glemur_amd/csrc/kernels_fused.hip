@@ -135,7 +135,8 @@ __device__ __forceinline__ double pdiff(const Pos &a, const Pos &b, int l) { ret
 // =============================================================== kernel
 // NT: threads = largest rect.  STAGE: 0..5, or 6 for the fields of the
 // current positions only (no slope written).  PRESS: pressure force on.
-template <int NT, int STAGE, bool PRESS>
+// PERI: periodic faces (the plain kernel is built without their code).
+template <int NT, int STAGE, bool PRESS, bool PERI = false>
 __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
 {
     constexpr bool FINAL = STAGE == 5;
@@ -168,17 +169,34 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
     const int t = threadIdx.x;
     const int c = t % RW, r = t / RW;
     const int gi = ix0 - 1 + c, gj = iy0 - 1 + r;
-    const bool col = t < RW * RH && gi >= 0 && gi <= g.nx + 1 && gj >= 0 && gj <= g.ny + 1;
-    const bool gx = gi == 0 || gi == g.nx + 1, gy = gj == 0 || gj == g.ny + 1;
-    const bool inB = col && c >= 1 && c <= RW - 2 && r >= 1 && r <= RH - 2 &&
+    // Periodic faces: a column beyond a periodic face stands for the inner
+    // column the reference copies into it (its source si, sj: periodic
+    // coordinates wrapped; src/boundaries.cu:450-582).  It holds that
+    // column's positions unshifted and computes that column's B, so the
+    // ghost's B is the copy, bit for bit.  A reader applies the period shift
+    // of the reference's ghost positions, (x - L) - h or (x + L) + h, to a
+    // neighbour that lies beyond a periodic face in the reader's own frame.
+    const bool px = PERI && g.xPeri, py = PERI && g.yPeri, pz = PERI && g.zPeri;
+    const int si = px ? (gi <= 0 ? gi + g.nx : gi >= g.nx + 1 ? gi - g.nx : gi) : gi;
+    const int sj = py ? (gj <= 0 ? gj + g.ny : gj >= g.ny + 1 ? gj - g.ny : gj) : gj;
+    const bool col = t < RW * RH && gi >= (px ? -1 : 0) && gi <= g.nx + (px ? 2 : 1) && gj >= (py ? -1 : 0) &&
+                     gj <= g.ny + (py ? 2 : 1);
+    // ghost columns / rows of line-tied faces (source frame)
+    const bool gx = si == 0 || si == g.nx + 1, gy = sj == 0 || sj == g.ny + 1;
+    const bool own_in = !PERI || (gi >= 0 && gi <= g.nx + 1 && gj >= 0 && gj <= g.ny + 1);   // own point padded
+    const bool inB = col && own_in && c >= 1 && c <= RW - 2 && r >= 1 && r <= RH - 2 &&
                      !((c == 1 || c == RW - 2) && (r == 1 || r == RH - 2)) && !(gx && gy);
     const bool inJ = col && c >= 2 && c <= RW - 3 && r >= 2 && r <= RH - 3 && gi >= 1 && gi <= ixe && gj >= 1 &&
                      gj <= iye;
-    const long long abase = (long long)gi + g.sx * (long long)gj;
+    const long long abase = (long long)gi + g.sx * (long long)gj;   // own point: stores
+    const long long asrc = (long long)si + g.sx * (long long)sj;    // source column: loads
     const int ii = gi - 1, jj = gj - 1;
-    // ghost face of an x/y ghost column
-    const int cface = gx ? 0 : 1, cside = gx ? (gi != 0) : (gj != 0);
-    const int cu = gx ? jj : ii;
+    // ghost face of an x/y ghost column (source coordinates)
+    const int cface = gx ? 0 : 1, cside = gx ? (si != 0) : (sj != 0);
+    const int cu = gx ? sj - 1 : si - 1;
+    // period shifts (-1, 0, +1) of the in-plane neighbour reads
+    const int shxm = px && si == 1 ? -1 : 0, shxp = px && si == g.nx ? 1 : 0;
+    const int shym = py && sj == 1 ? -1 : 0, shyp = py && sj == g.ny ? 1 : 0;
     const double eps_xy = (g.epsilonProf && inJ) ? A.epsx[ii] * A.epsy[jj] : 1.0;
     const R *Xh0 = A.X64[cur];
     const R *B0 = A.B0_64;
@@ -197,8 +215,25 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
     double dmax = 0, e_sum = 0, jmax = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
     int nan_seen = 0;
 
-    auto plane_ok = [&](int q) { return q >= max(Ks - 2, 0) && q <= min(Ke + 2, g.nz + 1); };
-    auto ghost_pt = [&](int q) { return gx || gy || q == 0 || q == g.nz + 1; };
+    // planes a chunk touches (one more beyond each end of a periodic z axis)
+    auto plane_ok = [&](int q) { return q >= max(Ks - 2, pz ? -1 : 0) && q <= min(Ke + 2, g.nz + (pz ? 2 : 1)); };
+    auto zgl = [&](int q) { return !pz && (q == 0 || q == g.nz + 1); };   // line-tied z ghost plane
+    auto ghost_pt = [&](int q) { return gx || gy || zgl(q); };
+    // source plane of a plane beyond a periodic z face, and the z shifts of
+    // reading plane q - 1 or q + 1 in the frame of plane q
+    auto zsrc = [&](int q) { return pz ? (q <= 0 ? q + g.nz : q >= g.nz + 1 ? q - g.nz : q) : q; };
+    auto zsh_m = [&](int q) { return pz && zsrc(q) == 1 ? -1 : 0; };
+    auto zsh_p = [&](int q) { return pz && zsrc(q) == g.nz ? 1 : 0; };
+    // a position read across a periodic face: component `axis` shifted by
+    // -+(L + h) in the reference's operation order
+    auto shift = [&](Pos &v, int axis, int sgn) {
+        if (PERI && sgn != 0) {
+            const double Lp = axis == 0 ? g.Lx : axis == 1 ? g.Ly : g.Lz;
+            const double h = axis == 0 ? g.dx : axis == 1 ? g.dy : g.dz;
+            v.x[axis] = sgn < 0 ? (v.x[axis] - Lp) - h : (v.x[axis] + Lp) + h;
+        }
+    };
+    (void)shxm; (void)shxp; (void)shym; (void)shyp; (void)zsh_m; (void)zsh_p; (void)shift;
     // ring slot of plane q (q > -NXL * 4 always holds here)
     auto XLslot = [&](int q) { return (unsigned)(q + NXL * 4) % (unsigned)NXL; };
     auto ld_x = [&](int q, int pt, Pos &o) {
@@ -219,7 +254,7 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
 
     // (0) loads of plane q into `in`
     auto load_plane = [&](int q, In &in) {
-        long long a = abase + g.sxy * (long long)q;
+        long long a = asrc + g.sxy * (long long)zsrc(q);
         bool gh = ghost_pt(q);
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
@@ -265,9 +300,8 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
         if (col && plane_ok(p + 2)) load_plane(p + 2, pf);
         // (2) B(p); ghost edges (two ghost coordinates) keep B0, as in the
         // reference, and are never read by J
-        if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1 &&
-            !((gx || gy) && (p == 0 || p == g.nz + 1))) {
-            const bool zg = p == 0 || p == g.nz + 1;
+        if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1 && !((gx || gy) && zgl(p))) {
+            const bool zg = zgl(p);
             R Bv[3];
             double detv = 0;
             if (!gx && !gy && !zg) {
@@ -276,6 +310,15 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                 ld_x(p, t + 1, xp);
                 ld_x(p, t - RW, ym);
                 ld_x(p, t + RW, yp);
+                Pos zp1 = xq[3], zm1 = xq[1];
+                if constexpr (PERI) {
+                    shift(xm, 0, shxm);
+                    shift(xp, 0, shxp);
+                    shift(ym, 1, shym);
+                    shift(yp, 1, shyp);
+                    shift(zp1, 2, zsh_p(p));
+                    shift(zm1, 2, zsh_m(p));
+                }
                 R jac[3][3], d[3];
 #pragma unroll
                 for (int l = 0; l < 3; ++l) d[l] = pdiff(xp, xm, l);
@@ -284,10 +327,23 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                 for (int l = 0; l < 3; ++l) d[l] = pdiff(yp, ym, l);
                 jac_col(jac, 1, d, (R)g.dy1);
 #pragma unroll
-                for (int l = 0; l < 3; ++l) d[l] = pdiff(xq[3], xq[1], l);
+                for (int l = 0; l < 3; ++l) d[l] = pdiff(zp1, zm1, l);
                 jac_col(jac, 2, d, (R)g.dz1);
                 double detJac1 = cauchy_b(jac, b0q[0], -1, Bv);
                 detv = 1. / detJac1;
+                if constexpr (PERI) {
+                    // a periodic ghost point's B (the copy of its source's)
+                    // for the fields of a dump; its det stays as the
+                    // reference leaves it
+                    if (A.emit && (gi == 0 || gi == g.nx + 1 || gj == 0 || gj == g.ny + 1 || p == 0 || p == g.nz + 1)) {
+                        int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
+                        if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
+                            long long a = abase + g.sxy * (long long)p;
+#pragma unroll
+                            for (int l = 0; l < 3; ++l) A.eB64[l * P + a] = Bv[l];
+                        }
+                    }
+                }
             } else {
                 // single ghost coordinate: line-tied face formula
                 int face, side, u, v;
@@ -295,8 +351,8 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                 if (zg) {
                     face = 2;
                     side = p != 0;
-                    u = ii;
-                    v = jj;
+                    u = si - 1;
+                    v = sj - 1;
                     const Pos &pin = side ? xq[1] : xq[3];
 #pragma unroll
                     for (int l = 0; l < 3; ++l) xin[l] = pin.x[l];
@@ -304,7 +360,7 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                     face = cface;
                     side = cside;
                     u = cu;
-                    v = p - 1;
+                    v = zsrc(p) - 1;
                     int off = face == 0 ? (side ? -1 : 1) : (side ? -RW : RW);
                     Pos pin;
                     ld_x(p, t + off, pin);
@@ -346,7 +402,15 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
             ld_x(pj, t + 1, xp);
             ld_x(pj, t - RW, ym);
             ld_x(pj, t + RW, yp);
-            const Pos &zm = xq[0], &zp = xq[2];
+            Pos zm = xq[0], zp = xq[2];
+            if constexpr (PERI) {
+                shift(xm, 0, shxm);
+                shift(xp, 0, shxp);
+                shift(ym, 1, shym);
+                shift(yp, 1, shyp);
+                shift(zm, 2, zsh_m(pj));
+                shift(zp, 2, zsh_p(pj));
+            }
             R Bxm[3], Bxp[3], Bym[3], Byp[3];
             ld_b(pj, t - 1, Bxm);
             ld_b(pj, t + 1, Bxp);
@@ -387,6 +451,11 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                         o = dk < 0 ? xq[0] : dk > 0 ? xq[2] : xq[1];
                     } else {
                         ld_x(pj + dk, t + di + dj * RW, o);
+                    }
+                    if constexpr (PERI) {
+                        shift(o, 0, di < 0 ? shxm : di > 0 ? shxp : 0);
+                        shift(o, 1, dj < 0 ? shym : dj > 0 ? shyp : 0);
+                        shift(o, 2, dk < 0 ? zsh_m(pj) : dk > 0 ? zsh_p(pj) : 0);
                     }
                 };
                 Pos pa, pb, pc, pd;
@@ -694,30 +763,69 @@ constexpr size_t lds_bytes64()
     return (size_t)(5 * 3 * NT + 3 * 3 * NT) * sizeof(double);
 }
 
-template <int NT, int STAGE, bool PRESS>
+template <int NT, int STAGE, bool PRESS, bool PERI>
 void launch_one(hipStream_t s, const FusedArgs &A)
 {
     static bool attr = false;
     constexpr size_t lds = lds_bytes64<NT>();
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_stage64<NT, STAGE, PRESS>,
+        (void)hipFuncSetAttribute((const void *)k_stage64<NT, STAGE, PRESS, PERI>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage64<NT, STAGE, PRESS>), dim3(A.nblocks), dim3(NT), lds, s, A);
+    hipLaunchKernelGGL((k_stage64<NT, STAGE, PRESS, PERI>), dim3(A.nblocks), dim3(NT), lds, s, A);
 }
 
-template <int NT, bool PRESS>
+template <int NT, bool PRESS, bool PERI>
 void launch_stage(hipStream_t s, const FusedArgs &A)
 {
     switch (A.stage) {
-    case 0: launch_one<NT, 0, PRESS>(s, A); break;
-    case 1: launch_one<NT, 1, PRESS>(s, A); break;
-    case 2: launch_one<NT, 2, PRESS>(s, A); break;
-    case 3: launch_one<NT, 3, PRESS>(s, A); break;
-    case 4: launch_one<NT, 4, PRESS>(s, A); break;
-    case 5: launch_one<NT, 5, PRESS>(s, A); break;
-    default: launch_one<NT, 6, PRESS>(s, A); break;
+    case 0: launch_one<NT, 0, PRESS, PERI>(s, A); break;
+    case 1: launch_one<NT, 1, PRESS, PERI>(s, A); break;
+    case 2: launch_one<NT, 2, PRESS, PERI>(s, A); break;
+    case 3: launch_one<NT, 3, PRESS, PERI>(s, A); break;
+    case 4: launch_one<NT, 4, PRESS, PERI>(s, A); break;
+    case 5: launch_one<NT, 5, PRESS, PERI>(s, A); break;
+    default: launch_one<NT, 6, PRESS, PERI>(s, A); break;
+    }
+}
+
+// Periodic ghost layers of the FP64 positions the controller left current
+// (buffer ctl->cur, read on the device so that graph replays follow the
+// controller's choice): the reference's setGridPeri (src/boundaries.cu:
+// 532-582), faces z, y, x in that order, as the per-kernel path launches it
+__global__ void k_set_grid_peri64(FusedArgs A, int face)
+{
+    const Geo &g = A.g;
+    double *X = A.X64[A.ctl->cur];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int na = face == 0 ? g.ny + 2 : g.nx + 2, nb = face == 2 ? g.ny + 2 : g.nz + 2;
+    if (t >= (long long)na * nb) return;
+    const int u = (int)(t % na), v = (int)(t / na);
+    const long long st[3] = {1, g.sx, g.sxy};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    int lo3[3], s3[3];
+    const int t0 = face == 0 ? 1 : 0, t1 = face == 2 ? 1 : 2;
+    lo3[t0] = s3[t0] = u;
+    lo3[t1] = s3[t1] = v;
+    long long lo = 0, hi = 0, srclo = 0, srchi = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (a == face) continue;
+        lo += lo3[a] * st[a];
+        srclo += s3[a] * st[a];
+    }
+    hi = lo + (long long)(nn[face] + 1) * st[face];
+    srchi = srclo + st[face];
+    srclo += (long long)nn[face] * st[face];
+    const double Lp = face == 0 ? g.Lx : face == 1 ? g.Ly : g.Lz, h = face == 0 ? g.dx : face == 1 ? g.dy : g.dz;
+    for (int l = 0; l < 3; ++l) {
+        double vlo = X[l * g.P + srclo], vhi = X[l * g.P + srchi];
+        if (l == face) {
+            vlo = vlo - Lp - h;
+            vhi = vhi + Lp + h;
+        }
+        X[l * g.P + lo] = vlo;
+        X[l * g.P + hi] = vhi;
     }
 }
 }  // namespace
@@ -759,8 +867,22 @@ void launch_fused(hipStream_t s, const FusedArgs &A, int f32k)
     if (f32k) {
         launch_fused32(s, A);
     } else {
-        if (pr) launch_stage<kNT64, true>(s, A);
-        else launch_stage<kNT64, false>(s, A);
+        const bool peri = A.g.xPeri || A.g.yPeri || A.g.zPeri;
+        if (pr) peri ? launch_stage<kNT64, true, true>(s, A) : launch_stage<kNT64, true, false>(s, A);
+        else peri ? launch_stage<kNT64, false, true>(s, A) : launch_stage<kNT64, false, false>(s, A);
+    }
+}
+
+void launch_set_grid_peri64(hipStream_t s, const FusedArgs &A)
+{
+    const Geo &g = A.g;
+    const int on[3] = {g.xPeri, g.yPeri, g.zPeri};
+    for (int face = 2; face >= 0; --face) {
+        if (!on[face]) continue;
+        const long long n = face == 0 ? (long long)(g.ny + 2) * (g.nz + 2)
+                            : face == 1 ? (long long)(g.nx + 2) * (g.nz + 2)
+                                        : (long long)(g.nx + 2) * (g.ny + 2);
+        hipLaunchKernelGGL(k_set_grid_peri64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A, face);
     }
 }
 

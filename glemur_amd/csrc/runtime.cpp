@@ -270,6 +270,7 @@ void enqueue_step(gm_ctx *c, int diag, int check_done)
     ev_end(c);
     // periodic ghost layers of the positions the controller left current
     if (f32k && any_peri(c->p)) gmd::launch_set_grid_peri32(c->stream, c->geo, c->Xh, c->Xl, c->ctl);
+    if (!f32k && any_peri(c->p)) gmd::launch_set_grid_peri64(c->stream, fused_args(c, 0, 0, 0, 0));
 }
 
 int sync_ctl(gm_ctx *c)
@@ -845,8 +846,6 @@ extern "C" int gm_reductions(gm_ctx *c, gm_reduction *red)
 // ------------------------------------------------------------ fused path
 static int check_fused(gm_ctx *c)
 {
-    if (any_peri(c->p) && c->prec != GM_PREC_F32K)
-        return fail(GM_ERR_UNSUPPORTED, "periodic boundaries of FP64 contexts run through the per-kernel path (gm_b_jacb0 ...)");
     if (!gm::starts(c->p.jMethod, "Stokes "))
         return fail(GM_ERR_UNSUPPORTED, "jMethod Classic / Stokes4th run through the per-kernel path (gm_current ...)");
     if (c->p.jxbAver) return fail(GM_ERR_UNSUPPORTED, "jxbAver runs through the per-kernel path (gm_rk_kk ...)");
@@ -854,13 +853,11 @@ static int check_fused(gm_ctx *c)
     return GM_OK;
 }
 
-// FP64 contexts the fused stage kernels do not cover (periodic faces,
-// jMethod = Classic or Stokes4th, jxbAver, inertia) step through the
-// per-kernel sequence
+// FP64 contexts the fused stage kernels do not cover (jMethod = Classic or
+// Stokes4th, jxbAver, inertia) step through the per-kernel sequence
 static bool per_kernel_path(gm_ctx *c)
 {
-    return c->prec == GM_PREC_F64 &&
-           (any_peri(c->p) || !gm::starts(c->p.jMethod, "Stokes ") || c->p.jxbAver || c->p.inertia);
+    return c->prec == GM_PREC_F64 && (!gm::starts(c->p.jMethod, "Stokes ") || c->p.jxbAver || c->p.inertia);
 }
 
 // Fields of the current state through the per-kernel sequence
@@ -883,6 +880,16 @@ extern "C" int gm_fields(gm_ctx *c)
     gmd::FusedArgs A = fused_args(c, -1, 1, 0, 0);
     gmd::launch_fused(c->stream, A, c->prec == GM_PREC_F32K);
     HIPCHK(hipGetLastError());
+    if (c->prec == GM_PREC_F64 && any_peri(c->p)) {
+        // B and detJac of every ghost face as the reference's updateBbound
+        // leaves them (its periodic branch evaluates a periodic ghost at the
+        // image plane, src/boundaries.cu:112-162), then B copied across the
+        // periodic faces (setPeri): the stage kernels need neither, the
+        // fields of a dump do
+        gmd::launch_bbound(c->stream, c->geo, c->X64[c->cur], c->B0_64, c->B64, c->det);
+        gmd::launch_set_peri(c->stream, c->geo, c->B64);
+        HIPCHK(hipGetLastError());
+    }
     return GM_OK;
 }
 

@@ -178,8 +178,8 @@ def test_batched_rows_from_step_log(require_gpu, tmp_path, prec):
 @pytest.mark.parametrize("faces", ["line-tied", "periodic"])
 def test_both_precisions_through_the_executable(require_gpu, tmp_path, faces):
     """The executable with all faces line-tied and with xPeri = zPeri = t
-    (FP64 periodic runs step on the per-kernel path, FP32 ones, -p 32, on
-    the fused stage kernels).  Both precisions write every row.  From dt0 =
+    (both precisions step on the fused stage kernels; FP32 with -p 32).
+    Both precisions write every row.  From dt0 =
     3e-4 the adaptive dt grows at a rate set by the error estimate, which
     FP32 rounding dominates while dt is small, so the two runs take their own
     step sequences and reach each row at their own t; compared at the same t

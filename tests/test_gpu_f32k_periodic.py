@@ -6,10 +6,10 @@ after every stage) and copies B of the opposite inner layer into it (setPeri,
 src/boundaries.cu:450-488).  The F32K stage kernels form those images at
 every stage from the opposite side's inputs, the ghost layer's B by the
 interior formula, which equals the copy in exact arithmetic.  Judged against
-the FP64 oracle (fields) and the FP64 per-kernel path, which equals the
-oracle bit for bit (trajectories), with the tolerances of the line-tied F32K
-tests; the run driver and graph replay reproduce step-by-step execution bit
-for bit."""
+the FP64 oracle (fields) and the FP64 context (trajectories; its fused
+kernels equal the oracle bit for bit), with the tolerances of the line-tied
+F32K tests; the run driver and graph replay reproduce step-by-step execution
+bit for bit."""
 import numpy as np
 import pytest
 
@@ -83,7 +83,7 @@ def test_periodic_f32k_fields_accuracy(g, case, tile):
 def test_periodic_f32k_tracks_fp64_at_fixed_dt(g, case):
     """25 steps at a fixed dt (0.3 of the dt the FP64 controller reaches in
     12 attempts, so every attempt is accepted in both precisions) in F32K and
-    on the FP64 per-kernel path: energy within 1e-6 relative, the relaxation
+    in FP64: energy within 1e-6 relative, the relaxation
     criterion within 1e-3, positions within 1e-6 of the displacement at every
     padded point (the periodic ghost layers kept by the F32K context after
     each step included); the F32K energy never rises."""
