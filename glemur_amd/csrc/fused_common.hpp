@@ -32,11 +32,14 @@ __device__ __forceinline__ int nface(const Geo &g, int face) { return face == 0 
 // x_in along the normal and the precomputed constants (src/boundaries.cu:
 // 164-186): jac = [normal column from x_in and the outer ghost | tangential
 // columns], B_normal = B0_normal.
-__device__ __forceinline__ void ghost_b(const Geo &g, const GhostPt &G, int face, int side, const double xin[3],
-                                       double B[3], double &det)
+// (FACE: the face as a compile-time constant, so that no array is indexed
+// at run time)
+template <int FACE>
+__device__ __forceinline__ void ghost_b_f(const Geo &g, const GhostPt &G, int side, const double xin[3],
+                                         double B[3], double &det)
 {
-    const double h1 = face == 0 ? g.dx1 : face == 1 ? g.dy1 : g.dz1;
-    const int t0 = face == 0 ? 1 : 0, t1 = face == 2 ? 1 : 2;
+    const double h1 = FACE == 0 ? g.dx1 : FACE == 1 ? g.dy1 : g.dz1;
+    constexpr int t0 = FACE == 0 ? 1 : 0, t1 = FACE == 2 ? 1 : 2;
     double jac[3][3], d[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
@@ -44,9 +47,16 @@ __device__ __forceinline__ void ghost_b(const Geo &g, const GhostPt &G, int face
         jac[l][t0] = G.tc0[l];
         jac[l][t1] = G.tc1[l];
     }
-    jac_col(jac, face, d, h1);
-    double detJac1 = cauchy_b(jac, G.b0, face, B);
+    jac_col(jac, FACE, d, h1);
+    double detJac1 = cauchy_b(jac, G.b0, FACE, B);
     det = 1 / detJac1;
+}
+__device__ __forceinline__ void ghost_b(const Geo &g, const GhostPt &G, int face, int side, const double xin[3],
+                                       double B[3], double &det)
+{
+    if (face == 0) ghost_b_f<0>(g, G, side, xin, B, det);
+    else if (face == 1) ghost_b_f<1>(g, G, side, xin, B, det);
+    else ghost_b_f<2>(g, G, side, xin, B, det);
 }
 
 }  // namespace

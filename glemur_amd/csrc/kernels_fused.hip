@@ -130,6 +130,54 @@ struct Pos {
 };
 __device__ __forceinline__ double pdiff(const Pos &a, const Pos &b, int l) { return a.x[l] - b.x[l]; }
 
+// B and det at a ghost point of line-tied x (F = 0) or y (F = 1) faces with
+// the constants of k_ghost_setup formed by the same operations: tangential
+// neighbours um, up (in-plane) and vm, vp (z) from the ghost layer, or
+// rebuilt as x_g -+ h where the reference's launch tiles synthesise them
+// (e.. flags: its 16-wide tile edges and the last row); the outer ghost
+// x_g + h n; B0 of the point
+template <int F>
+__device__ __forceinline__ void face_ghost_b(const Geo &g, const Pos &xg, Pos um, Pos up, Pos vm, Pos vp, bool eum,
+                                             bool eup, bool evm, bool evp, const double b0[3], int side,
+                                             const double xin[3], double B[3], double &det)
+{
+    constexpr int t0 = F == 0 ? 1 : 0;
+    const double ht0 = F == 0 ? g.dy : g.dx, h1t0 = F == 0 ? g.dy1 : g.dx1;
+    if (eum) {
+        um = xg;
+        um.x[t0] = xg.x[t0] + (double)(-1) * ht0;
+    }
+    if (eup) {
+        up = xg;
+        up.x[t0] = xg.x[t0] + (double)(+1) * ht0;
+    }
+    if (evm) {
+        vm = xg;
+        vm.x[2] = xg.x[2] + (double)(-1) * g.dz;
+    }
+    if (evp) {
+        vp = xg;
+        vp.x[2] = xg.x[2] + (double)(+1) * g.dz;
+    }
+    GhostPt G;
+    double jg[3][3], d[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) d[l] = up.x[l] - um.x[l];
+    jac_col(jg, t0, d, h1t0);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) d[l] = vp.x[l] - vm.x[l];
+    jac_col(jg, 2, d, (double)g.dz1);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        G.tc0[l] = jg[l][t0];
+        G.tc1[l] = jg[l][2];
+        G.x2[l] = xg.x[l];
+        G.b0[l] = b0[l];
+    }
+    G.x2[F] = xg.x[F] + (2 * side - 1) * (F == 0 ? g.dx : g.dy);
+    ghost_b_f<F>(g, G, side, xin, B, det);
+}
+
 }  // namespace
 
 // =============================================================== kernel
@@ -367,9 +415,25 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
 #pragma unroll
                     for (int l = 0; l < 3; ++l) xin[l] = pin.x[l];
                 }
-                const GhostPt &G = A.ghost[A.ghost_off[face * 2 + side] + u + (long long)nface(g, face) * v];
                 double Bd[3];
-                ghost_b(g, G, face, side, xin, Bd, detv);
+                if (zg) {
+                    const GhostPt G = A.ghost[A.ghost_off[face * 2 + side] + u + (long long)nface(g, face) * v];
+                    ghost_b_f<2>(g, G, side, xin, Bd, detv);
+                } else {
+                    // x and y faces: the constants k_ghost_setup tabulates,
+                    // formed from the ghost layer's positions this thread
+                    // already holds (no table load in the march)
+                    const int toff = face == 0 ? RW : 1;
+                    Pos um, up;
+                    ld_x(p, t - toff, um);   // in the rect; unused where rebuilt
+                    ld_x(p, t + toff, up);
+                    const bool eum = u % 16 == 0, eup = u % 16 == 15 || u == (face == 0 ? g.ny : g.nx) - 1;
+                    const bool evm = v % 16 == 0, evp = v % 16 == 15 || v == g.nz - 1;
+                    if (face == 0)
+                        face_ghost_b<0>(g, xq[2], um, up, xq[1], xq[3], eum, eup, evm, evp, b0q[0], side, xin, Bd, detv);
+                    else
+                        face_ghost_b<1>(g, xq[2], um, up, xq[1], xq[3], eum, eup, evm, evp, b0q[0], side, xin, Bd, detv);
+                }
 #pragma unroll
                 for (int l = 0; l < 3; ++l) Bv[l] = (R)Bd[l];
                 if (A.emit) {
