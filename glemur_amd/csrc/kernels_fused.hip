@@ -130,53 +130,6 @@ struct Pos {
 };
 __device__ __forceinline__ double pdiff(const Pos &a, const Pos &b, int l) { return a.x[l] - b.x[l]; }
 
-// B and det at a ghost point of line-tied x (F = 0) or y (F = 1) faces with
-// the constants of k_ghost_setup formed by the same operations: tangential
-// neighbours um, up (in-plane) and vm, vp (z) from the ghost layer, or
-// rebuilt as x_g -+ h where the reference's launch tiles synthesise them
-// (e.. flags: its 16-wide tile edges and the last row); the outer ghost
-// x_g + h n; B0 of the point
-template <int F>
-__device__ __forceinline__ void face_ghost_b(const Geo &g, const Pos &xg, Pos um, Pos up, Pos vm, Pos vp, bool eum,
-                                             bool eup, bool evm, bool evp, const double b0[3], int side,
-                                             const double xin[3], double B[3], double &det)
-{
-    constexpr int t0 = F == 0 ? 1 : 0;
-    const double ht0 = F == 0 ? g.dy : g.dx, h1t0 = F == 0 ? g.dy1 : g.dx1;
-    if (eum) {
-        um = xg;
-        um.x[t0] = xg.x[t0] + (double)(-1) * ht0;
-    }
-    if (eup) {
-        up = xg;
-        up.x[t0] = xg.x[t0] + (double)(+1) * ht0;
-    }
-    if (evm) {
-        vm = xg;
-        vm.x[2] = xg.x[2] + (double)(-1) * g.dz;
-    }
-    if (evp) {
-        vp = xg;
-        vp.x[2] = xg.x[2] + (double)(+1) * g.dz;
-    }
-    GhostPt G;
-    double jg[3][3], d[3];
-#pragma unroll
-    for (int l = 0; l < 3; ++l) d[l] = up.x[l] - um.x[l];
-    jac_col(jg, t0, d, h1t0);
-#pragma unroll
-    for (int l = 0; l < 3; ++l) d[l] = vp.x[l] - vm.x[l];
-    jac_col(jg, 2, d, (double)g.dz1);
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-        G.tc0[l] = jg[l][t0];
-        G.tc1[l] = jg[l][2];
-        G.x2[l] = xg.x[l];
-        G.b0[l] = b0[l];
-    }
-    G.x2[F] = xg.x[F] + (2 * side - 1) * (F == 0 ? g.dx : g.dy);
-    ghost_b_f<F>(g, G, side, xin, B, det);
-}
 
 }  // namespace
 
@@ -240,7 +193,7 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
     const long long asrc = (long long)si + g.sx * (long long)sj;    // source column: loads
     const int ii = gi - 1, jj = gj - 1;
     // ghost face of an x/y ghost column (source coordinates)
-    const int cface = gx ? 0 : 1, cside = gx ? (si != 0) : (sj != 0);
+    const int cside = gx ? (si != 0) : (sj != 0);
     const int cu = gx ? sj - 1 : si - 1;
     // period shifts (-1, 0, +1) of the in-plane neighbour reads
     const int shxm = px && si == 1 ? -1 : 0, shxp = px && si == g.nx ? 1 : 0;
@@ -352,7 +305,15 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
             const bool zg = zgl(p);
             R Bv[3];
             double detv = 0;
-            if (!gx && !gy && !zg) {
+            if (!zg) {
+                // The interior formula, which line-tied x and y ghost points
+                // share: such a point's face formula (src/boundaries.cu:
+                // 164-186, 287-335) is the same Jacobian with the outer
+                // neighbour along the normal replaced by the outer ghost
+                // x_g + h n and the tangential ones by x_g -+ h where the
+                // reference's 16-wide launch tiles synthesise them (their
+                // edges and the last row), B's normal component held at B0.
+                // Taking one path keeps their waves from running two
                 Pos xm, xp, ym, yp;
                 ld_x(p, t - 1, xm);
                 ld_x(p, t + 1, xp);
@@ -367,6 +328,45 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                     shift(zp1, 2, zsh_p(p));
                     shift(zm1, 2, zsh_m(p));
                 }
+                if (gx || gy) {
+                    const Pos &xg = xq[2];
+                    const int v = zsrc(p) - 1;
+                    if (gx) {
+                        Pos x2 = xg;
+                        x2.x[0] = xg.x[0] + (2 * cside - 1) * g.dx;
+                        if (cside) xp = x2;
+                        else xm = x2;
+                        if (cu % 16 == 0) {
+                            ym = xg;
+                            ym.x[1] = xg.x[1] + (double)(-1) * g.dy;
+                        }
+                        if (cu % 16 == 15 || cu == g.ny - 1) {
+                            yp = xg;
+                            yp.x[1] = xg.x[1] + (double)(+1) * g.dy;
+                        }
+                    } else {
+                        Pos x2 = xg;
+                        x2.x[1] = xg.x[1] + (2 * cside - 1) * g.dy;
+                        if (cside) yp = x2;
+                        else ym = x2;
+                        if (cu % 16 == 0) {
+                            xm = xg;
+                            xm.x[0] = xg.x[0] + (double)(-1) * g.dx;
+                        }
+                        if (cu % 16 == 15 || cu == g.nx - 1) {
+                            xp = xg;
+                            xp.x[0] = xg.x[0] + (double)(+1) * g.dx;
+                        }
+                    }
+                    if (v % 16 == 0) {
+                        zm1 = xg;
+                        zm1.x[2] = xg.x[2] + (double)(-1) * g.dz;
+                    }
+                    if (v % 16 == 15 || v == g.nz - 1) {
+                        zp1 = xg;
+                        zp1.x[2] = xg.x[2] + (double)(+1) * g.dz;
+                    }
+                }
                 R jac[3][3], d[3];
 #pragma unroll
                 for (int l = 0; l < 3; ++l) d[l] = pdiff(xp, xm, l);
@@ -377,63 +377,32 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
 #pragma unroll
                 for (int l = 0; l < 3; ++l) d[l] = pdiff(zp1, zm1, l);
                 jac_col(jac, 2, d, (R)g.dz1);
-                double detJac1 = cauchy_b(jac, b0q[0], -1, Bv);
+                double detJac1 = cauchy_b(jac, b0q[0], gx ? 0 : gy ? 1 : -1, Bv);
                 detv = 1. / detJac1;
-                if constexpr (PERI) {
-                    // a periodic ghost point's B (the copy of its source's)
-                    // for the fields of a dump; its det stays as the
-                    // reference leaves it
-                    if (A.emit && (gi == 0 || gi == g.nx + 1 || gj == 0 || gj == g.ny + 1 || p == 0 || p == g.nz + 1)) {
-                        int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
-                        if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
-                            long long a = abase + g.sxy * (long long)p;
+                // a ghost point's B (and a line-tied one's det) for the
+                // fields of a dump; a periodic ghost's det stays as the
+                // reference's updateBbound leaves it
+                if (A.emit && (gx || gy || (PERI && (gi == 0 || gi == g.nx + 1 || gj == 0 || gj == g.ny + 1 || p == 0 ||
+                                                     p == g.nz + 1)))) {
+                    int ci = min(max(gi, 1), g.nx) - 1, cj = min(max(gj, 1), g.ny) - 1, ck = min(max(p, 1), g.nz);
+                    if (ci >= ix0 && ci < ixe && cj >= iy0 && cj < iye && ck >= Ks && ck <= Ke) {
+                        long long a = abase + g.sxy * (long long)p;
 #pragma unroll
-                            for (int l = 0; l < 3; ++l) A.eB64[l * P + a] = Bv[l];
-                        }
+                        for (int l = 0; l < 3; ++l) A.eB64[l * P + a] = Bv[l];
+                        if (gx || gy) A.edet[a] = detv;
                     }
                 }
             } else {
-                // single ghost coordinate: line-tied face formula
-                int face, side, u, v;
+                // z faces: the tabulated face constants (the initial
+                // distortion enters through them)
+                const int face = 2, side = p != 0, u = si - 1, v = sj - 1;
                 double xin[3];
-                if (zg) {
-                    face = 2;
-                    side = p != 0;
-                    u = si - 1;
-                    v = sj - 1;
-                    const Pos &pin = side ? xq[1] : xq[3];
+                const Pos &pin = side ? xq[1] : xq[3];
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) xin[l] = pin.x[l];
-                } else {
-                    face = cface;
-                    side = cside;
-                    u = cu;
-                    v = zsrc(p) - 1;
-                    int off = face == 0 ? (side ? -1 : 1) : (side ? -RW : RW);
-                    Pos pin;
-                    ld_x(p, t + off, pin);
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) xin[l] = pin.x[l];
-                }
+                for (int l = 0; l < 3; ++l) xin[l] = pin.x[l];
                 double Bd[3];
-                if (zg) {
-                    const GhostPt G = A.ghost[A.ghost_off[face * 2 + side] + u + (long long)nface(g, face) * v];
-                    ghost_b_f<2>(g, G, side, xin, Bd, detv);
-                } else {
-                    // x and y faces: the constants k_ghost_setup tabulates,
-                    // formed from the ghost layer's positions this thread
-                    // already holds (no table load in the march)
-                    const int toff = face == 0 ? RW : 1;
-                    Pos um, up;
-                    ld_x(p, t - toff, um);   // in the rect; unused where rebuilt
-                    ld_x(p, t + toff, up);
-                    const bool eum = u % 16 == 0, eup = u % 16 == 15 || u == (face == 0 ? g.ny : g.nx) - 1;
-                    const bool evm = v % 16 == 0, evp = v % 16 == 15 || v == g.nz - 1;
-                    if (face == 0)
-                        face_ghost_b<0>(g, xq[2], um, up, xq[1], xq[3], eum, eup, evm, evp, b0q[0], side, xin, Bd, detv);
-                    else
-                        face_ghost_b<1>(g, xq[2], um, up, xq[1], xq[3], eum, eup, evm, evp, b0q[0], side, xin, Bd, detv);
-                }
+                const GhostPt G = A.ghost[A.ghost_off[face * 2 + side] + u + (long long)nface(g, face) * v];
+                ghost_b_f<2>(g, G, side, xin, Bd, detv);
 #pragma unroll
                 for (int l = 0; l < 3; ++l) Bv[l] = (R)Bd[l];
                 if (A.emit) {
