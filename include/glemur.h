@@ -268,7 +268,7 @@ int gm_batch_run(gm_batch *b, const gm_run_opts *opts, gm_run_stats *stats);
  * whole ensemble for each candidate policy (every member's fewest-lines or
  * fewest-points rect within 512, 768 or 1024 points, shared z-chunk
  * length) on the current states and keeps the fastest (a later candidate
- * must be more than 1% faster to replace an earlier one); ms_best (6
+ * must be more than 2% faster to replace an earlier one); ms_best (6
  * entries, or NULL) receives the kept policies' mean times.  States, t and
  * dt unchanged. */
 int gm_batch_autotune(gm_batch *b, int reps, double ms_best[6]);
@@ -362,7 +362,7 @@ int gm_set_stage_tile(gm_ctx *ctx, int stage, int tx, int ty, int cz);
  * `reps` launches of each candidate rect (the fewest-loads rect for each
  * workgroup size the kernel is built for, with one or two rounds of
  * workgroups over the CUs) on the current state and keeps the fastest (a
- * later candidate must be more than 1% faster to replace an earlier one);
+ * later candidate must be more than 2% faster to replace an earlier one);
  * ms_best (6 entries, or NULL) receives the kept tiles' mean times.  The
  * accepted state, t and dt are unchanged. */
 int gm_autotune(gm_ctx *ctx, int reps, double ms_best[6]);
