@@ -299,6 +299,19 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
         }
         // (0) loads of plane p+2, in flight while B and J are computed
         if (col && plane_ok(p + 2)) load_plane(p + 2, pf);
+        // final stage: the five slopes of the output point (plane p-1),
+        // issued here so that the B section covers their latency
+        double kpre[5][3];
+        (void)kpre;
+        if constexpr (FINAL) {
+            if (inJ && p - 1 >= Ks && p - 1 <= Ke) {
+                const long long a = abase + g.sxy * (long long)(p - 1);
+#pragma unroll
+                for (int m = 0; m < 5; ++m)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) kpre[m][l] = A.K64[m * P3 + l * P + a];
+            }
+        }
         // (2) B(p); ghost edges (two ghost coordinates) keep B0, as in the
         // reference, and are never read by J
         if (inB && p >= Ks - 1 && p <= Ke + 1 && p >= 0 && p <= g.nz + 1 && !((gx || gy) && zgl(p))) {
@@ -583,7 +596,7 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                 for (int l = 0; l < 3; ++l) {
                     double k6[6];
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) k6[m] = A.K64[m * P3 + l * P + a];
+                    for (int m = 0; m < 5; ++m) k6[m] = kpre[m][l];
                     k6[5] = kv[l];
                     double x6 = xq[1].x[l];
 #pragma unroll
