@@ -191,7 +191,9 @@ def ensemble(g, workloads, n, steps, autotune=True):
     for m, p in zip(b.members, ps):
         B0, xb = g.init_state(p)
         m.upload(B0, xb, 0.0, 0.01)
-    tuned = b.autotune(reps=3) if autotune else None
+    # twelve launches per candidate: the stage-0 candidates differ by about
+    # 1-2%, within the noise of a few launches
+    tuned = b.autotune(reps=12) if autotune else None
     # launch configuration per stage: the largest workgroup a member's rect
     # needs (at least 512 threads) and the members' workgroups together
     launches = []
