@@ -431,8 +431,8 @@ class Context:
 
 class Batch:
     """An ensemble of relaxations advanced together (gm_batch_*, FP32 with
-    compensated positions).  Members are Context views: upload, download and
-    tiles per member; run() advances all of them."""
+    compensated positions or FP64).  Members are Context views: upload,
+    download and tiles per member; run() advances all of them."""
 
     def __init__(self, params_list, precision=PREC_F32K):
         self.params = list(params_list)

@@ -18,6 +18,22 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return base + w;
 }
 
+// The inverse of xcd_remap: the workgroup index that xcd_remap sends to
+// logical index L among nb.
+__device__ __forceinline__ int xcd_unremap(int L, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, head = r * (q + 1);
+    int x, w;
+    if (L < head) {
+        x = L / (q + 1);
+        w = L - x * (q + 1);
+    } else {   // q > 0 here: L < nb = 8q + r
+        x = r + (L - head) / q;
+        w = L - head - (x - r) * q;
+    }
+    return w * 8 + x;
+}
+
 // Two-sum (Knuth): s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
 {

@@ -91,6 +91,9 @@ void launch_controller(hipStream_t s, const CtlArgs &A);
 int batch_threads32(int rect);
 void launch_fused32_batch(hipStream_t s, int stage, int press, int nt, const FusedArgs *args, const int *start,
                           int n, int total);
+// The FP64 stage kernel over n line-tied instances (512-thread workgroups).
+void launch_fused64_batch(hipStream_t s, int stage, int press, const FusedArgs *args, const int *start, int n,
+                          int total);
 void launch_controller_batch(hipStream_t s, const CtlArgs *args, int n);
 
 }  // namespace gmd

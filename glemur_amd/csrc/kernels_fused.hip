@@ -137,8 +137,13 @@ __device__ __forceinline__ double pdiff(const Pos &a, const Pos &b, int l) { ret
 // NT: threads = largest rect.  STAGE: 0..5, or 6 for the fields of the
 // current positions only (no slope written).  PRESS: pressure force on.
 // PERI: periodic faces (the plain kernel is built without their code).
-template <int NT, int STAGE, bool PRESS, bool PERI = false>
-__global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
+// One workgroup's march: `L` is its logical index in the instance (after
+// the XCD-aware remap), which picks the tile and z chunk; `pb` is the slot
+// of its partial results (the hardware workgroup index of a standalone
+// launch, so that the controller sums the partials in one order whether
+// the instance runs alone or in an ensemble).
+template <int NT, int STAGE, bool PRESS, bool PERI>
+__device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L, const int pb)
 {
     constexpr bool FINAL = STAGE == 5;
     using R = double;
@@ -159,7 +164,6 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
     constexpr int n = STAGE == 6 ? 0 : STAGE;
     const long long P = g.P, P3 = 3 * P;
 
-    const int L = xcd_remap(blockIdx.x, A.nblocks);
     const int tx = L % A.tiles_x, ty = (L / A.tiles_x) % A.tiles_y, ch = L / (A.tiles_x * A.tiles_y);
     const int ix0 = tx * A.TX, iy0 = ty * A.TY;
     const int Ks = ch * A.CZ + 1, Ke = min(ch * A.CZ + A.CZ, g.nz);
@@ -662,7 +666,7 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
         if (FINAL) {
             double m = 0;
             for (int q = 0; q < NT / 64; ++q) m = nb_max(m, red[0][q]);
-            A.part_delta[blockIdx.x] = m;
+            A.part_delta[pb] = m;
         }
         if (A.diag) {
             double s = 0, a[5] = {0, 0, 0, 0, 0};
@@ -670,10 +674,34 @@ __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
                 s += red[1][q];
                 for (int m = 0; m < 5; ++m) a[m] = nb_max(a[m], red[m + 2][q]);
             }
-            A.part_diag[blockIdx.x] = s;
-            for (int m = 0; m < 5; ++m) A.part_diag[(m + 1) * A.nblocks + blockIdx.x] = a[m];
+            A.part_diag[pb] = s;
+            for (int m = 0; m < 5; ++m) A.part_diag[(m + 1) * A.nblocks + pb] = a[m];
         }
     }
+}
+
+template <int NT, int STAGE, bool PRESS, bool PERI = false>
+__global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
+{
+    stage64_body<NT, STAGE, PRESS, PERI>(A, xcd_remap(blockIdx.x, A.nblocks), blockIdx.x);
+}
+
+// Ensemble: one launch covers the workgroups of n instances; start[i] is
+// the first (remapped) workgroup of instance i, start[n] the total.  Each
+// instance's partials go to the slots its standalone launch would use.
+template <int NT, int STAGE, bool PRESS>
+__global__ void __launch_bounds__(NT) k_stage64_batch(const FusedArgs *__restrict__ args, const int *__restrict__ start,
+                                                      int n)
+{
+    const int L = xcd_remap(blockIdx.x, start[n]);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {   // last instance whose first workgroup is <= L
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= L) lo = mid;
+        else hi = mid - 1;
+    }
+    const int lb = L - start[lo];
+    stage64_body<NT, STAGE, PRESS, false>(args[lo], lb, xcd_unremap(lb, args[lo].nblocks));
 }
 
 // ============================================================ controller
@@ -822,6 +850,32 @@ void launch_one(hipStream_t s, const FusedArgs &A)
     hipLaunchKernelGGL((k_stage64<NT, STAGE, PRESS, PERI>), dim3(A.nblocks), dim3(NT), lds, s, A);
 }
 
+template <int NT, int STAGE, bool PRESS>
+void launch_one_batch(hipStream_t s, const FusedArgs *args, const int *start, int n, int total)
+{
+    static bool attr = false;
+    constexpr size_t lds = lds_bytes64<NT>();
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_stage64_batch<NT, STAGE, PRESS>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_stage64_batch<NT, STAGE, PRESS>), dim3(total), dim3(NT), lds, s, args, start, n);
+}
+
+template <int NT, bool PRESS>
+void launch_stage_batch(hipStream_t s, int stage, const FusedArgs *args, const int *start, int n, int total)
+{
+    switch (stage) {
+    case 0: launch_one_batch<NT, 0, PRESS>(s, args, start, n, total); break;
+    case 1: launch_one_batch<NT, 1, PRESS>(s, args, start, n, total); break;
+    case 2: launch_one_batch<NT, 2, PRESS>(s, args, start, n, total); break;
+    case 3: launch_one_batch<NT, 3, PRESS>(s, args, start, n, total); break;
+    case 4: launch_one_batch<NT, 4, PRESS>(s, args, start, n, total); break;
+    default: launch_one_batch<NT, 5, PRESS>(s, args, start, n, total); break;
+    }
+}
+
 template <int NT, bool PRESS, bool PERI>
 void launch_stage(hipStream_t s, const FusedArgs &A)
 {
@@ -917,6 +971,13 @@ void launch_fused(hipStream_t s, const FusedArgs &A, int f32k)
         if (pr) peri ? launch_stage<kNT64, true, true>(s, A) : launch_stage<kNT64, true, false>(s, A);
         else peri ? launch_stage<kNT64, false, true>(s, A) : launch_stage<kNT64, false, false>(s, A);
     }
+}
+
+void launch_fused64_batch(hipStream_t s, int stage, int press, const FusedArgs *args, const int *start, int n,
+                          int total)
+{
+    if (press) launch_stage_batch<kNT64, true>(s, stage, args, start, n, total);
+    else launch_stage_batch<kNT64, false>(s, stage, args, start, n, total);
 }
 
 void launch_set_grid_peri64(hipStream_t s, const FusedArgs &A)

@@ -1367,6 +1367,7 @@ struct gm_batch {
     std::vector<gm_ctx *> inst;
     hipStream_t stream = nullptr;
     int press = 0;
+    bool f64 = false;               // FP64 members (the FP64 stage kernel), else F32K
     gmd::FusedArgs *args[6] = {};   // per stage: one argument block per member
     int *start[6] = {};             // per stage: first workgroup of each member, then the total
     int total[6] = {};
@@ -1381,12 +1382,32 @@ struct gm_batch {
 namespace {
 
 // Workgroup size of a batch stage launch (k = 6: the fields pass, which
-// members run on their own): the largest any member's rect needs
+// members run on their own): the largest any member's rect needs (the FP64
+// stage kernel always runs 512-thread workgroups)
 int batch_nt(const gm_batch *b, int k)
 {
+    if (b->f64) return gmd::fused_max_points(0);
     int nt = 512;
     for (const gm_ctx *c : b->inst) nt = std::max(nt, gmd::batch_threads32((c->TX[k] + 4) * (c->TY[k] + 4)));
     return nt;
+}
+
+// Largest member rect a batch stage launch holds.
+int batch_max_rect(const gm_batch *b, int k)
+{
+    return b->f64 ? gmd::fused_max_points(0) : gmd::max_rect32(k, b->press);
+}
+
+// Resident workgroups per CU of a batch stage launch in `nt`-thread workgroups.
+int batch_blocks_per_cu(const gm_batch *b, int nt, int k)
+{
+    return b->f64 ? 1 : gmd::blocks_per_cu32(nt, k == 6 ? -1 : k, b->press);
+}
+
+void batch_launch(const gm_batch *b, int s, int nt, const gmd::FusedArgs *args, const int *start, int n, int total)
+{
+    if (b->f64) gmd::launch_fused64_batch(b->stream, s, b->press, args, start, n, total);
+    else gmd::launch_fused32_batch(b->stream, s, b->press, nt, args, start, n, total);
 }
 
 void batch_free_dev(gm_batch *b)
@@ -1403,8 +1424,9 @@ int batch_prepare(gm_batch *b)
     batch_free_dev(b);
     for (int s = 0; s < 6; ++s) {
         b->nt[s] = batch_nt(b, s);
-        if (b->nt[s] > gmd::max_rect32(s, b->press))
-            return fail(GM_ERR_ARG, "ensemble member tile exceeds the stage kernel's workgroup");
+        bool fits = b->nt[s] <= batch_max_rect(b, s);
+        for (const gm_ctx *c : b->inst) fits = fits && (c->TX[s] + 4) * (c->TY[s] + 4) <= b->nt[s];
+        if (!fits) return fail(GM_ERR_ARG, "ensemble member tile exceeds the stage kernel's workgroup");
     }
     std::vector<gmd::FusedArgs> A(n);
     std::vector<int> st(n + 1);
@@ -1431,7 +1453,7 @@ int batch_prepare(gm_batch *b)
         K.nb_delta = c->nblocks[5];
         K.nb_diag = c->nblocks[0];
         K.diag = 1;
-        K.nbuf = 2;
+        K.nbuf = b->f64 ? 3 : 2;
         K.maxError = c->p.maxError;
         K.dtMin = c->p.dtMin;
         K.cell = c->p.dx * c->p.dy * c->p.dz;
@@ -1448,7 +1470,7 @@ void batch_enqueue_step(gm_batch *b)
     const int n = (int)b->inst.size();
     for (int s = 0; s < 6; ++s) {
         b->tm.begin(b->stream, s);
-        gmd::launch_fused32_batch(b->stream, s, b->press, b->nt[s], b->args[s], b->start[s], n, b->total[s]);
+        batch_launch(b, s, b->nt[s], b->args[s], b->start[s], n, b->total[s]);
         b->tm.end(b->stream);
     }
     b->tm.begin(b->stream, 6);
@@ -1466,7 +1488,7 @@ static void batch_chunks_k(gm_batch *b, int k)
 {
     gm_ctx *c0 = b->inst[0];
     {
-        const long long slots = (long long)c0->ncu * gmd::blocks_per_cu32(batch_nt(b, k), k == 6 ? -1 : k, b->press);
+        const long long slots = (long long)c0->ncu * batch_blocks_per_cu(b, batch_nt(b, k), k);
         int maxnz = 1;
         for (gm_ctx *c : b->inst) maxnz = std::max(maxnz, c->p.nz);
         int bestL = maxnz;
@@ -1489,7 +1511,7 @@ static void batch_chunks(gm_batch *b)
 extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm_batch **out)
 {
     if (!params || !out || n < 1) return fail(GM_ERR_ARG, "gm_batch_create: bad arguments");
-    if (precision != GM_PREC_F32K) return fail(GM_ERR_UNSUPPORTED, "ensembles run the FP32 stage kernel (GM_PREC_F32K)");
+    if (precision != GM_PREC_F32K && precision != GM_PREC_F64) return fail(GM_ERR_ARG, "unknown precision");
     for (int i = 0; i < n; ++i) {
         if (params[i].pressure != params[0].pressure)
             return fail(GM_ERR_ARG, "ensemble members must agree on pressure (one kernel variant per launch)");
@@ -1505,12 +1527,16 @@ extern "C" int gm_batch_create(const gm_params *params, int n, int precision, gm
         gm_ctx *c = nullptr;
         if ((r = gm_create(&params[i], precision, &c))) return bail(r);
         b->inst.push_back(c);
+        if (per_kernel_path(c))
+            return bail(fail(GM_ERR_UNSUPPORTED, "ensemble members: jMethod Classic / Stokes4th, jxbAver and inertia "
+                                                 "run the per-kernel path"));
         c->zrev = false;   // the members together move several GB per stage
         c->zalt = false;
         auto_tiles(c, std::max(1, (c->ncu + n - 1) / n), true);
         b->ctlh.push_back(c->ctlh);
     }
     b->press = params[0].pressure ? 1 : 0;
+    b->f64 = precision == GM_PREC_F64;
     batch_chunks(b);
     for (gm_ctx *c : b->inst)
         if ((r = ensure_partials(c))) return bail(r);
@@ -1652,7 +1678,7 @@ extern "C" int gm_batch_autotune(gm_batch *b, int reps, double ms_best[6])
         double best = 1e30;
         for (int i = 0; i < n; ++i) { bTX[i] = b->inst[i]->TX[s]; bTY[i] = b->inst[i]->TY[s]; bCZ[i] = b->inst[i]->CZ[s]; }
         for (int w = 0; w < 3 && rc == GM_OK; ++w) {
-            if (bounds[w] > gmd::max_rect32(s, b->press)) break;
+            if (bounds[w] > batch_max_rect(b, s)) break;
             for (int m = 0; m < 2 && rc == GM_OK; ++m) {
                 for (gm_ctx *c : b->inst) {
                     int tx = c->TX[s], ty = c->TY[s];
@@ -1673,10 +1699,9 @@ extern "C" int gm_batch_autotune(gm_batch *b, int reps, double ms_best[6])
                     break;
                 }
                 const int nt = batch_nt(b, s);
-                gmd::launch_fused32_batch(b->stream, s, b->press, nt, dargs, dstart, n, st[n]);   // warm
+                batch_launch(b, s, nt, dargs, dstart, n, st[n]);   // warm
                 if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(GM_ERR_HIP, "event"); break; }
-                for (int q = 0; q < reps; ++q)
-                    gmd::launch_fused32_batch(b->stream, s, b->press, nt, dargs, dstart, n, st[n]);
+                for (int q = 0; q < reps; ++q) batch_launch(b, s, nt, dargs, dstart, n, st[n]);
                 if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                     rc = fail(GM_ERR_HIP, "gm_batch_autotune: launch");
                     break;

@@ -252,12 +252,16 @@ int gm_step_log(gm_ctx *ctx, gm_step_record *out, int n);
 
 /* ---- ensembles (no reference counterpart) ----------------------------- */
 /* n independent relaxations (sizes and parameters may differ; pressure must
- * agree) advanced together in FP32 with compensated positions: each stage is
- * one launch over the workgroups of all members, one controller launch
- * applies every member's accept/reject, dt and stop rules, and finished
- * members drop out of every later launch.  Members are ordinary contexts
- * (gm_batch_context) for upload, download, step logs and tiles; they belong
- * to the ensemble and are destroyed with it. */
+ * agree; line-tied faces) advanced together, in FP32 with compensated
+ * positions (GM_PREC_F32K) or in FP64 (GM_PREC_F64, the fused FP64 stage
+ * kernel: jMethod Stokes, no jxbAver, no inertia): each stage is one launch
+ * over the workgroups of all members, one controller launch applies every
+ * member's accept/reject, dt and stop rules, and finished members drop out
+ * of every later launch.  A member's results equal its standalone
+ * context's bit for bit (with the same tiles; FP64 results do not depend on
+ * the tiles).  Members are ordinary contexts (gm_batch_context) for upload,
+ * download, step logs and tiles; they belong to the ensemble and are
+ * destroyed with it. */
 typedef struct gm_batch gm_batch;
 int gm_batch_create(const gm_params *params, int n, int precision, gm_batch **out);
 int gm_batch_destroy(gm_batch *b);
@@ -266,7 +270,8 @@ int gm_batch_context(gm_batch *b, int i, gm_ctx **ctx);
 int gm_batch_run(gm_batch *b, const gm_run_opts *opts, gm_run_stats *stats);
 /* Ensemble tile autotuner: per stage kernel, times `reps` launches of the
  * whole ensemble for each candidate policy (every member's fewest-lines or
- * fewest-points rect within 512, 768 or 1024 points, shared z-chunk
+ * fewest-points rect within 512, 768 or 1024 points -- 512 for FP64
+ * members -- shared z-chunk
  * length) on the current states and keeps the fastest (a later candidate
  * must be more than 2% faster to replace an earlier one); ms_best (6
  * entries, or NULL) receives the kept policies' mean times.  States, t and

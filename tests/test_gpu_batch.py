@@ -1,8 +1,10 @@
 """Ensembles (gm_batch_*): every stage of every member in one launch, one
 controller launch for all members.  Each member must equal a standalone
-FP32 context run with the same tile bit for bit (same stage kernel, same
-per-point arithmetic, same reduction order), members stop on their own
-limits, and graph replay equals stream launches."""
+context of its precision (F32K or FP64) run with the same tile bit for bit
+(same stage kernel, same per-point arithmetic, same reduction order),
+members stop on their own limits, and graph replay equals stream launches.
+FP64 members also follow the oracle bit for bit, as standalone FP64
+contexts do."""
 import numpy as np
 import pytest
 
@@ -127,3 +129,106 @@ def test_default_tiles_mix_workgroup_sizes(g):
         for r, q in zip(got[i][2], log):
             assert (r.maxDelta, r.dt_used, r.accepted, r.ratio) == (q.maxDelta, q.dt_used, q.accepted, q.ratio), i
             assert abs(r.energy - q.energy) <= 1e-12 * abs(q.energy), i
+
+
+# ------------------------------------------------------------------ FP64
+
+def f64_members(**kw):
+    """Three small FP64 braids of different sizes (one with the epsilon
+    profile)."""
+    ps = [small_braid(19, 17, 23, **kw), small_braid(18, 16, 31, **kw), small_braid(21, 18, 17, epsilonProf=True, **kw)]
+    for p in ps:
+        p.nCycle = 10 ** 6
+    return ps
+
+
+def log_rows(ctx, n):
+    return [(r.maxDelta, r.dt_used, r.accepted, r.energy, r.ratio, r.jmax) for r in ctx.step_log(n)]
+
+
+@pytest.mark.parametrize("press", [False, True], ids=["line_tied", "pressure"])
+def test_f64_members_equal_standalone_runs(g, press):
+    """FP64 members equal their standalone FP64 contexts run with the same
+    tile bit for bit: positions, t, dt and every step-log row, the energy
+    sum included (a member's block partials go to the slots of its
+    standalone launch, so the controller sums them in the same order)."""
+    ps = f64_members(**({"pressure": True, "beta": 0.05} if press else {}))
+    states = [g.init_state(p) for p in ps]
+    with g.Batch(ps, g.PREC_F64) as b:
+        for m, (B0, xb) in zip(b.members, states):
+            m.set_tile(*TILE)
+            m.upload(B0, xb, 0.0, 0.01)
+        stats = b.run(max_attempts=16, batch=8)
+        got = [(m.download(g.FIELD_XB), m.time(), log_rows(m, 16)) for m in b.members]
+    for i, (p, (B0, xb)) in enumerate(zip(ps, states)):
+        with g.Context(p, g.PREC_F64) as c:
+            c.set_tile(*TILE)
+            c.upload(B0, xb, 0.0, 0.01)
+            st = c.run(max_attempts=16, batch=8)
+            want = (c.download(g.FIELD_XB), c.time(), log_rows(c, 16))
+        assert stats[i].n_attempt == st.n_attempt == 16 and stats[i].n_accept == st.n_accept, i
+        assert got[i][1] == want[1], i
+        assert got[i][2] == want[2], i
+        assert np.array_equal(got[i][0], want[0]), i
+
+
+def test_f64_members_follow_the_oracle(g):
+    """Every FP64 member, attempt by attempt with the default ensemble tiles:
+    error maximum, dt used and decision equal the oracle's run on the
+    device's dt sequence (dt itself within 2 ulp: device and host pow), and
+    the positions after eight attempts equal the oracle's bit for bit."""
+    import oracle as o
+    ps = f64_members()
+    states = [g.init_state(p) for p in ps]
+    sims = [o.Sim(o.from_product(p), B0, xb) for p, (B0, xb) in zip(ps, states)]
+    with g.Batch(ps, g.PREC_F64) as b:
+        for m, (B0, xb) in zip(b.members, states):
+            m.upload(B0, xb, 0.0, 0.02)
+        for it in range(8):
+            for m, sim in zip(b.members, sims):
+                sim.set_dt(m.time()[1])
+            b.run(max_attempts=1, batch=1)
+            for i, (m, sim) in enumerate(zip(b.members, sims)):
+                acc, dtu, md, _ = sim.step()
+                r = m.step_log(1)[0]
+                assert (r.maxDelta, r.dt_used, bool(r.accepted)) == (md, dtu, acc), (i, it)
+                assert abs(m.time()[1] - sim.dt) <= 2 * np.spacing(sim.dt), (i, it)
+        for i, (m, sim) in enumerate(zip(b.members, sims)):
+            assert np.array_equal(m.download(g.FIELD_XB), sim.arr("xb")), i
+
+
+def test_f64_autotuned_ensemble_and_graph_replay(g):
+    """FP64 results do not depend on the tile: after the ensemble autotuner
+    (512-point policies) the members' positions, t and decisions equal those
+    of an ensemble on its default tiles, and a graph-replayed run equals
+    stream launches."""
+    ps = f64_members(pressure=True, beta=0.05)
+    states = [g.init_state(p) for p in ps]
+    res = []
+    for tune, use_graph in ((False, False), (True, False), (True, True)):
+        with g.Batch(ps, g.PREC_F64) as b:
+            for m, (B0, xb) in zip(b.members, states):
+                m.upload(B0, xb, 0.0, 0.01)
+            if tune:
+                ms = b.autotune(reps=2)
+                assert len(ms) == 6 and all(t > 0 for t in ms)
+                for m in b.members:
+                    assert all((tx + 4) * (ty + 4) <= 512 for tx, ty, _, _ in (m.stage_tile(s) for s in range(6)))
+            for _ in range(2):
+                b.run(max_attempts=6, batch=6, use_graph=use_graph)
+            res.append([(m.time(), m.download(g.FIELD_XB), [r[:3] for r in log_rows(m, 12)]) for m in b.members])
+    for other in res[1:]:
+        for (ta, xa, la), (tb, xb2, lb) in zip(res[0], other):
+            assert ta == tb and la == lb and np.array_equal(xa, xb2)
+
+
+def test_f64_ensemble_refuses_per_kernel_members(g):
+    """Members that need the FP64 per-kernel path (inertia, jMethod Classic)
+    are refused, as are periodic faces."""
+    for kw in ({"inertia": True, "nu": 0.6}, {"xPeri": True}):
+        with pytest.raises(g.GlemurError):
+            g.Batch([small_braid(), small_braid(**kw)], g.PREC_F64)
+    p = small_braid()
+    p.set_str("jMethod", "Classic")
+    with pytest.raises(g.GlemurError):
+        g.Batch([small_braid(), p], g.PREC_F64)
