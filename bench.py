@@ -26,7 +26,8 @@ max|JxB| / max(|J||B|) <= 1e-3.
 
 ensemble: 64 seeded braids of the sizes 33x33x97 ... 97x97x289 advanced
 together (one launch per stage for all members); aggregate grid-point
-RK-stages per second.
+RK-stages per second.  ensemble_f64: the same members in FP64 (the fused
+FP64 stage kernel, bitwise with the oracle), 10 attempts each.
 
 large: the 257 x 257 x 769 braid (seeded amplitudes, pressure on) run
 for 2000 accepted F32K steps from its initial state; aggregate grid-point
@@ -181,13 +182,16 @@ def relaxation_probe(g, p, B0, xb, seconds):
             "energy": st.energy, "time_to_relax_s": sec if st.converged else None}
 
 
-def ensemble(g, workloads, n, steps, autotune=True):
+def ensemble(g, workloads, n, steps, autotune=True, prec=None):
     """The 64-member ensemble: tiles from the ensemble autotuner, `steps`
     attempts per member timed as a whole, then per-launch event timing of
-    the stage launches over further attempts for the kernel roofline."""
+    the stage launches over further attempts for the kernel roofline.  F32K
+    (the c4 workload) by default; FP64 members with prec = PREC_F64."""
+    prec = g.PREC_F32K if prec is None else prec
+    f32k = prec == g.PREC_F32K
     ps = [workloads.ensemble_member(i) for i in range(n)]
     pts = sum(p.nx * p.ny * p.nz for p in ps)
-    b = g.Batch(ps)
+    b = g.Batch(ps, prec)
     for m, p in zip(b.members, ps):
         B0, xb = g.init_state(p)
         m.upload(B0, xb, 0.0, 0.01)
@@ -199,10 +203,11 @@ def ensemble(g, workloads, n, steps, autotune=True):
     launches = []
     for st in range(6):
         tl = [m.stage_tile(st) for m in b.members]
-        launches.append((max(512, max(threads32((t[0] + 4) * (t[1] + 4)) for t in tl)), sum(t[3] for t in tl)))
+        nt = max(512, max(threads32((t[0] + 4) * (t[1] + 4)) for t in tl)) if f32k else 512
+        launches.append((nt, sum(t[3] for t in tl)))
     pv = profile_values(TRAFFIC_C4)
     prof_launch = [tuple(int(x) for x in pv.get(f"launch_ensemble_stage{st}", ())) for st in range(6)]
-    traffic = pv.get("traffic_bytes_per_launch_ensemble", (None,))[0] if prof_launch == launches else None
+    traffic = pv.get("traffic_bytes_per_launch_ensemble", (None,))[0] if f32k and prof_launch == launches else None
     b.run(max_attempts=3, batch=3, n_cycle=10 ** 9)             # warm-up
     t0 = time.perf_counter()
     st = b.run(max_attempts=steps, batch=steps, n_cycle=10 ** 9)
@@ -216,21 +221,24 @@ def ensemble(g, workloads, n, steps, autotune=True):
     sizes = sorted({(p.nx, p.ny, p.nz) for p in ps})
     per_stage = [ms[k] / max(1, cnt[k]) for k in range(6)]
     mean_ms = sum(ms[:6]) / max(1, sum(cnt[:6]))
-    achieved = 80.0 * pts / (mean_ms * 1e-3) / 1e9
-    return {"members": n, "inner_points": pts, "sizes": [list(s) for s in sizes],
+    bpp = g.algorithmic_bytes(prec)
+    achieved = bpp * pts / (mean_ms * 1e-3) / 1e9
+    return {"members": n, "precision": "F32K" if f32k else "FP64", "inner_points": pts,
+            "sizes": [list(s) for s in sizes],
             "attempts_per_member": steps, "seconds": sec,
             "value": sum(p.nx * p.ny * p.nz * 6 * (s.n_attempt - 3) for p, s in zip(ps, st)) / sec,
             "unit": "grid-pt RK-stages/s", "attempts_total": int(att),
             "tiles": "ensemble autotuner (gm_batch_autotune)" if autotune else "default",
             "autotune_stage_ms": tuned,
-            "roofline": {"kernel": "k_stage32_batch (one launch per stage over all members; workgroup size, "
+            "roofline": {"kernel": ("k_stage32_batch" if f32k else "k_stage64_batch") +
+                                   " (one launch per stage over all members; workgroup size, "
                                    "workgroups per stage: " + ", ".join(f"{a}x{b}" for a, b in launches) + ")",
                          "traffic": traffic,
                          "traffic_source": os.path.relpath(TRAFFIC_C4, ROOT) if traffic else None,
                          "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "unit": "GB/s",
                          "mean_launch_ms": mean_ms, "per_stage_ms": per_stage,
                          "controller_ms": ms[6] / max(1, cnt[6]),
-                         "algorithmic_bytes_per_launch": 80.0 * pts}}
+                         "algorithmic_bytes_per_launch": bpp * pts}}
 
 
 def time_to_relax(g, theta):
@@ -420,6 +428,7 @@ def main():
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--relax-seconds", type=float, default=10.0)
     ap.add_argument("--ensemble", type=int, default=64, help="ensemble members (0: skip)")
+    ap.add_argument("--ensemble-f64", type=int, default=64, help="FP64 ensemble members (0: skip)")
     ap.add_argument("--ttr", type=float, default=0.023,
                     help="time-to-relax threshold for the cold/warm-start comparison (0: skip)")
     ap.add_argument("--large-steps", type=int, default=2000,
@@ -511,6 +520,8 @@ def main():
         out["relaxation"] = relaxation_probe(g, p, B0, xb, args.relax_seconds)
     if args.ensemble > 0:
         out["ensemble"] = ensemble(g, workloads, args.ensemble, 20)
+    if args.ensemble_f64 > 0:
+        out["ensemble_f64"] = ensemble(g, workloads, args.ensemble_f64, 10, prec=g.PREC_F64)
     if args.large_steps > 0:
         out["large"] = large_run(g, workloads, args.large_steps)
     if args.cli:
