@@ -138,12 +138,12 @@ __device__ __forceinline__ double pdiff(const Pos &a, const Pos &b, int l) { ret
 // current positions only (no slope written).  PRESS: pressure force on.
 // PERI: periodic faces (the plain kernel is built without their code).
 // One workgroup's march: `L` is its logical index in the instance (after
-// the XCD-aware remap), which picks the tile and z chunk; `pb` is the slot
-// of its partial results (the hardware workgroup index of a standalone
-// launch, so that the controller sums the partials in one order whether
-// the instance runs alone or in an ensemble).
+// the XCD-aware remap), which picks the tile and z chunk.  Its partial
+// results go to slot xcd_unremap(L), the hardware workgroup index of a
+// standalone launch, so that the controller sums the partials in one order
+// whether the instance runs alone or in an ensemble.
 template <int NT, int STAGE, bool PRESS, bool PERI>
-__device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L, const int pb)
+__device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L)
 {
     constexpr bool FINAL = STAGE == 5;
     using R = double;
@@ -662,6 +662,7 @@ __device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L, co
         }
     }
     __syncthreads();
+    const int pb = xcd_unremap(L, A.nblocks);
     if (threadIdx.x == 0) {
         if (FINAL) {
             double m = 0;
@@ -683,7 +684,7 @@ __device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L, co
 template <int NT, int STAGE, bool PRESS, bool PERI = false>
 __global__ void __launch_bounds__(NT) k_stage64(FusedArgs A)
 {
-    stage64_body<NT, STAGE, PRESS, PERI>(A, xcd_remap(blockIdx.x, A.nblocks), blockIdx.x);
+    stage64_body<NT, STAGE, PRESS, PERI>(A, xcd_remap(blockIdx.x, A.nblocks));
 }
 
 // Ensemble: one launch covers the workgroups of n instances; start[i] is
@@ -700,8 +701,7 @@ __global__ void __launch_bounds__(NT) k_stage64_batch(const FusedArgs *__restric
         if (start[mid] <= L) lo = mid;
         else hi = mid - 1;
     }
-    const int lb = L - start[lo];
-    stage64_body<NT, STAGE, PRESS, false>(args[lo], lb, xcd_unremap(lb, args[lo].nblocks));
+    stage64_body<NT, STAGE, PRESS, false>(args[lo], L - start[lo]);
 }
 
 // ============================================================ controller
