@@ -258,8 +258,9 @@ int gm_step_log(gm_ctx *ctx, gm_step_record *out, int n);
  * over the workgroups of all members, one controller launch applies every
  * member's accept/reject, dt and stop rules, and finished members drop out
  * of every later launch.  A member's results equal its standalone
- * context's bit for bit (with the same tiles; FP64 results do not depend on
- * the tiles).  Members are ordinary contexts (gm_batch_context) for upload,
+ * context's bit for bit with the same tiles and z chunks (FP64 positions
+ * and decisions with any tiles; the energy sum's order follows the
+ * chunks).  Members are ordinary contexts (gm_batch_context) for upload,
  * download, step logs and tiles; they belong to the ensemble and are
  * destroyed with it. */
 typedef struct gm_batch gm_batch;
