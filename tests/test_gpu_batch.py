@@ -64,23 +64,34 @@ def test_members_equal_standalone_runs(g):
         assert np.array_equal(got[i][0], want[0]), i
 
 
-def test_members_stop_on_their_own_limits(g):
+@pytest.mark.parametrize("prec", ["F32K", "F64"])
+def test_members_stop_on_their_own_limits(g, prec):
     """Each member stops after its own nCycle accepted steps; finished
-    members drop out of the remaining launches (their state stays put)."""
+    members drop out of the remaining launches: the first to finish (5
+    steps, while the others run to 20) ends where a standalone context of
+    its precision stops after 5 steps, bit for bit."""
+    pr = getattr(g, "PREC_" + prec)
     ps = members(g)
     for p, n in zip(ps, (5, 12, 20, 9)):
         p.nCycle = n
-    with g.Batch(ps) as b:
+    with g.Batch(ps, pr) as b:
         for m, p in zip(b.members, ps):
             B0, xb = g.init_state(p)
             m.upload(B0, xb, 0.0, 0.01)
         stats = b.run(batch=4)
         for s, p in zip(stats, ps):
             assert s.n_accept == p.nCycle and s.stopped and not s.converged
+        x0, t0 = b.members[0].download(g.FIELD_XB), b.members[0].time()
         # a further run advances every member by its own nCycle again
         stats2 = b.run(batch=4, n_cycle=0)
         for s, s2, p in zip(stats, stats2, ps):
             assert s2.n_accept == s.n_accept + p.nCycle
+    with g.Context(ps[0], pr) as c:
+        B0, xb = g.init_state(ps[0])
+        c.upload(B0, xb, 0.0, 0.01)
+        st = c.run(batch=4)
+        assert st.n_accept == 5 and c.time() == t0
+        assert np.array_equal(c.download(g.FIELD_XB), x0)
 
 
 def test_graph_replay_equals_stream(g):
