@@ -19,13 +19,17 @@ def g(require_gpu):
     return glemur_amd
 
 
-def members(g):
-    """Four small braids of different sizes and blob sets."""
-    ps = [small_braid(19, 17, 23), small_braid(18, 16, 31), small_braid(21, 18, 17, epsilonProf=True)]
+def members(g, pressure=False):
+    """Four small braids of different sizes and blob sets (optionally all
+    with the pressure term)."""
+    kw = {"pressure": True, "beta": 0.05} if pressure else {}
+    ps = [small_braid(19, 17, 23, **kw), small_braid(18, 16, 31, **kw), small_braid(21, 18, 17, epsilonProf=True, **kw)]
     from glemur_amd import workloads
     q = workloads.braid_params(20, 20, 41, n_cycle=10 ** 6)
     g.braid_seeded(q, 1003, 2, 8, True)
     q.Lx, q.Ly, q.Lz, q.Ox, q.Oy, q.Oz = 4.0, 4.0, 16.0, -2.0, -2.0, -8.0
+    if pressure:
+        q.pressure, q.beta = True, 0.05
     g.params_derive(q)
     ps.append(q)
     for p in ps:
@@ -36,12 +40,14 @@ def members(g):
 TILE = (18, 16, 9)   # rects of 21..22 x 20 points (> 384): 512-thread workgroups in both paths
 
 
-def test_members_equal_standalone_runs(g):
-    """Each ensemble member equals its standalone run bit for bit.  The
+@pytest.mark.parametrize("press", [False, True], ids=["line_tied", "pressure"])
+def test_members_equal_standalone_runs(g, press):
+    """Each ensemble member equals its standalone run bit for bit, without
+    and with the pressure term (the pressure stage kernels).  The
     standalone F32K contexts march every odd stage down the grid, the
     members always upwards, so this also shows that the march direction
     does not change a bit of the result."""
-    ps = members(g)
+    ps = members(g, press)
     states = [g.init_state(p) for p in ps]
     with g.Batch(ps) as b:
         for m, (B0, xb) in zip(b.members, states):
