@@ -64,6 +64,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_braid129x129x385_hbm_traf
 
 TRAFFIC_C3 = os.path.join(ROOT, "profiles", "k_stage32_pressure_braid257x257x769_hbm_traffic.md")
 TRAFFIC_C4 = os.path.join(ROOT, "profiles", "k_stage32_batch_ensemble64_hbm_traffic.md")
+TRAFFIC_C4_F64 = os.path.join(ROOT, "profiles", "k_stage64_batch_ensemble64_hbm_traffic.md")
 
 
 def profile_values(path):
@@ -205,9 +206,11 @@ def ensemble(g, workloads, n, steps, autotune=True, prec=None):
         tl = [m.stage_tile(st) for m in b.members]
         nt = max(512, max(threads32((t[0] + 4) * (t[1] + 4)) for t in tl)) if f32k else 512
         launches.append((nt, sum(t[3] for t in tl)))
-    pv = profile_values(TRAFFIC_C4)
-    prof_launch = [tuple(int(x) for x in pv.get(f"launch_ensemble_stage{st}", ())) for st in range(6)]
-    traffic = pv.get("traffic_bytes_per_launch_ensemble", (None,))[0] if f32k and prof_launch == launches else None
+    tag = "ensemble" if f32k else "ensemble_f64"
+    prof = TRAFFIC_C4 if f32k else TRAFFIC_C4_F64
+    pv = profile_values(prof)
+    prof_launch = [tuple(int(x) for x in pv.get(f"launch_{tag}_stage{st}", ())) for st in range(6)]
+    traffic = pv.get(f"traffic_bytes_per_launch_{tag}", (None,))[0] if prof_launch == launches else None
     b.run(max_attempts=3, batch=3, n_cycle=10 ** 9)             # warm-up
     t0 = time.perf_counter()
     st = b.run(max_attempts=steps, batch=steps, n_cycle=10 ** 9)
@@ -234,7 +237,7 @@ def ensemble(g, workloads, n, steps, autotune=True, prec=None):
                                    " (one launch per stage over all members; workgroup size, "
                                    "workgroups per stage: " + ", ".join(f"{a}x{b}" for a, b in launches) + ")",
                          "traffic": traffic,
-                         "traffic_source": os.path.relpath(TRAFFIC_C4, ROOT) if traffic else None,
+                         "traffic_source": os.path.relpath(prof, ROOT) if traffic else None,
                          "achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "unit": "GB/s",
                          "mean_launch_ms": mean_ms, "per_stage_ms": per_stage,
                          "controller_ms": ms[6] / max(1, cnt[6]),
@@ -521,7 +524,9 @@ def main():
     if args.ensemble > 0:
         out["ensemble"] = ensemble(g, workloads, args.ensemble, 20)
     if args.ensemble_f64 > 0:
-        out["ensemble_f64"] = ensemble(g, workloads, args.ensemble_f64, 10, prec=g.PREC_F64)
+        # default tiles: the FP64 autotuner's stage-0 pick between two
+        # policies within 1% changes from run to run
+        out["ensemble_f64"] = ensemble(g, workloads, args.ensemble_f64, 10, autotune=False, prec=g.PREC_F64)
     if args.large_steps > 0:
         out["large"] = large_run(g, workloads, args.large_steps)
     if args.cli:
