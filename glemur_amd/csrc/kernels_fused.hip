@@ -142,7 +142,7 @@ __device__ __forceinline__ double pdiff(const Pos &a, const Pos &b, int l) { ret
 // results go to slot xcd_unremap(L), the hardware workgroup index of a
 // standalone launch, so that the controller sums the partials in one order
 // whether the instance runs alone or in an ensemble.
-template <int NT, int STAGE, bool PRESS, bool PERI>
+template <int NT, int STAGE, bool PRESS, bool PERI, bool KPRE = true>
 __device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L)
 {
     constexpr bool FINAL = STAGE == 5;
@@ -304,10 +304,12 @@ __device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L)
         // (0) loads of plane p+2, in flight while B and J are computed
         if (col && plane_ok(p + 2)) load_plane(p + 2, pf);
         // final stage: the five slopes of the output point (plane p-1),
-        // issued here so that the B section covers their latency
+        // issued here so that the B section covers their latency (KPRE; the
+        // ensemble build, whose registers do not take them, loads them
+        // where they are used)
         double kpre[5][3];
         (void)kpre;
-        if constexpr (FINAL) {
+        if constexpr (FINAL && KPRE) {
             if (inJ && p - 1 >= Ks && p - 1 <= Ke) {
                 const long long a = abase + g.sxy * (long long)(p - 1);
 #pragma unroll
@@ -600,7 +602,7 @@ __device__ __forceinline__ void stage64_body(const FusedArgs &A, const int L)
                 for (int l = 0; l < 3; ++l) {
                     double k6[6];
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) k6[m] = kpre[m][l];
+                    for (int m = 0; m < 5; ++m) k6[m] = KPRE ? kpre[m][l] : A.K64[m * P3 + l * P + a];
                     k6[5] = kv[l];
                     double x6 = xq[1].x[l];
 #pragma unroll
@@ -701,7 +703,7 @@ __global__ void __launch_bounds__(NT) k_stage64_batch(const FusedArgs *__restric
         if (start[mid] <= L) lo = mid;
         else hi = mid - 1;
     }
-    stage64_body<NT, STAGE, PRESS, false>(args[lo], L - start[lo]);
+    stage64_body<NT, STAGE, PRESS, false, false>(args[lo], L - start[lo]);
 }
 
 // ============================================================ controller
